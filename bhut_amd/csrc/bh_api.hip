@@ -1,0 +1,769 @@
+// bh_api.hip -- context, caching allocator and the C ABI of libbhut_hip.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+
+#include "bh_common.h"
+
+namespace bh {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  return e == hipErrorOutOfMemory ? BH_ENOMEM : BH_EHIP;
+}
+
+static size_t round_size(size_t b) {
+  if (b < (1u << 20)) return (b + 255) & ~size_t(255);
+  return (b + (size_t(1) << 16) - 1) & ~((size_t(1) << 16) - 1);
+}
+
+static void trim_locked(bh_ctx* c) {
+  for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+  c->free_blocks.clear();
+  c->cached_bytes = 0;
+}
+
+// Device allocations are cached per context and reused by size: every
+// kernel runs on the context stream, so a block released by one call and
+// handed to the next is ordered behind the kernels that used it.
+void* dalloc(bh_ctx* c, size_t bytes) {
+  const size_t rs = round_size(bytes);
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->free_blocks.lower_bound(rs);
+  if (it != c->free_blocks.end() && it->first <= rs + rs / 4) {
+    void* p = it->second;
+    c->live[p] = it->first;
+    c->cached_bytes -= it->first;
+    c->free_blocks.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, rs);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(c->stream);
+    trim_locked(c);
+    e = hipMalloc(&p, rs);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("device allocation of %zu bytes failed", rs);
+      return nullptr;
+    }
+  }
+  c->live[p] = rs;
+  return p;
+}
+
+void dfree(bh_ctx* c, void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->live.find(p);
+  if (it == c->live.end()) return;
+  c->free_blocks.emplace(it->second, p);
+  c->cached_bytes += it->second;
+  c->live.erase(it);
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+#define BH_REQ(cond, ...)          \
+  do {                             \
+    if (!(cond)) {                 \
+      ::bh::set_error(__VA_ARGS__); \
+      return BH_EINVAL;            \
+    }                              \
+  } while (0)
+
+#define BH_TRY(expr)           \
+  do {                         \
+    int _rc = (expr);          \
+    if (_rc != BH_OK) return _rc; \
+  } while (0)
+
+static void free_tree_arrays(bh_tree* t) {
+  bh_ctx* c = t->ctx;
+  void* ps[] = {t->spart, t->perm,  t->start, t->count, t->first_child, t->parent, t->skip,
+                t->leaf_order, t->center, t->half, t->mass, t->com, t->bmin, t->bmax, t->gbox,
+                t->nodes, t->spart32};
+  for (void* p : ps) dfree(c, p);
+}
+
+template <class T>
+static bool alloc_into(bh_ctx* c, T*& p, size_t n) {
+  p = dalloc_t<T>(c, n);
+  return p != nullptr;
+}
+
+static int alloc_topology(bh_tree* t) {
+  bh_ctx* c = t->ctx;
+  const size_t M = (size_t)t->M;
+  if (!alloc_into(c, t->start, M) || !alloc_into(c, t->count, M) ||
+      !alloc_into(c, t->first_child, M) || !alloc_into(c, t->parent, M) ||
+      !alloc_into(c, t->skip, M) || !alloc_into(c, t->leaf_order, (size_t)t->nleaves) ||
+      !alloc_into(c, t->center, M * t->dim) || !alloc_into(c, t->half, M) ||
+      !alloc_into(c, t->mass, M) || !alloc_into(c, t->com, M * t->dim) ||
+      !alloc_into(c, t->bmin, M * t->dim) || !alloc_into(c, t->bmax, M * t->dim) ||
+      !alloc_into(c, t->nodes, M))
+    return BH_ENOMEM;
+  return BH_OK;
+}
+
+static int build_dev(bh_ctx* c, const double* pos, const double* mass, int64_t n, int dim,
+                     int leaf, bh_tree* t) {
+  t->ctx = c;
+  t->n = n;
+  t->dim = dim;
+  t->leaf = leaf;
+  Scratch s(c);
+  if (!alloc_into(c, t->gbox, 2 * (size_t)dim) || !alloc_into(c, t->spart, (size_t)n) ||
+      !alloc_into(c, t->perm, (size_t)n))
+    return BH_ENOMEM;
+  uint64_t* k0 = s.get<uint64_t>(n);
+  uint64_t* k1 = s.get<uint64_t>(n);
+  uint32_t* v1 = s.get<uint32_t>(n);
+  if (!k0 || !k1 || !v1) return BH_ENOMEM;
+  uint32_t* v0 = reinterpret_cast<uint32_t*>(t->perm);
+  BH_TRY(bbox(c, pos, n, dim, t->gbox));
+  BH_TRY(morton_iota(c, pos, n, dim, t->gbox, 0, k0, v0));
+  uint64_t* ko;
+  uint32_t* vo;
+  BH_TRY(radix_sort(c, k0, v0, n, 21 * dim, &ko, &vo, k1, v1));
+  if (vo != v0) BH_HIP(hipMemcpyAsync(v0, vo, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+  BH_TRY(gather(c, pos, mass, v0, n, dim, t->spart));
+  // the (N, leaf) tables give M before the node arrays are allocated
+  BH_TRY(topology_plan(n, leaf, &t->M, &t->nleaves, &t->depth));
+  t->binary = true;
+  BH_TRY(alloc_topology(t));
+  BH_TRY(topology(c, t));
+  BH_TRY(upward(c, t, 0));
+  BH_TRY(pack_nodes(c, t));
+  return BH_OK;
+}
+
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" {
+
+const char* bh_last_error(void) { return g_err.c_str(); }
+
+int bh_version(void) { return 1; }
+
+int bh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int bh_ctx_create(int device, bh_ctx** out) {
+  BH_REQ(out, "out is NULL");
+  *out = nullptr;
+  int n = bh_device_count();
+  if (n <= 0) {
+    set_error("no HIP device is visible");
+    return BH_ENODEV;
+  }
+  BH_REQ(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
+  DeviceGuard g(device);
+  bh_ctx* c = new bh_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+  }
+  c->stream = c->own_stream;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return BH_OK;
+}
+
+void bh_ctx_destroy(bh_ctx* c) {
+  if (!c) return;
+  DeviceGuard g(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    trim_locked(c);
+    for (auto& kv : c->live) (void)hipFree(kv.first);
+    c->live.clear();
+  }
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+int bh_ctx_synchronize(bh_ctx* c) {
+  BH_REQ(c, "ctx is NULL");
+  DeviceGuard g(c->device);
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+void* bh_ctx_stream(bh_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int bh_ctx_set_stream(bh_ctx* c, void* stream) {
+  BH_REQ(c, "ctx is NULL");
+  DeviceGuard g(c->device);
+  BH_HIP(hipStreamSynchronize(c->stream));
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return BH_OK;
+}
+
+int bh_ctx_trim(bh_ctx* c) {
+  BH_REQ(c, "ctx is NULL");
+  DeviceGuard g(c->device);
+  BH_HIP(hipStreamSynchronize(c->stream));
+  std::lock_guard<std::mutex> lk(c->mu);
+  trim_locked(c);
+  return BH_OK;
+}
+
+// ------------------------------------------------------------------ build
+int bh_tree_build_dev(bh_ctx* c, const double* d_pos, const double* d_mass, int64_t n, int32_t dim,
+                      int32_t leaf_size, bh_tree** out) {
+  BH_REQ(c && out, "NULL argument");
+  *out = nullptr;
+  BH_REQ(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  BH_REQ(leaf_size > 0, "leaf_size must be positive, got %d", leaf_size);
+  if (n <= 0) {
+    set_error("Cannot build tree with zero particles");
+    return BH_EEMPTY;
+  }
+  BH_REQ(n < 0x7fffffffll, "n = %lld exceeds the int32 index range", (long long)n);
+  BH_REQ(d_pos && d_mass, "NULL positions or masses");
+  DeviceGuard g(c->device);
+  bh_tree* t = new bh_tree();
+  int rc = build_dev(c, d_pos, d_mass, n, dim, leaf_size, t);
+  if (rc == BH_OK) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "build", __FILE__, __LINE__);
+  }
+  if (rc != BH_OK) {
+    free_tree_arrays(t);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return BH_OK;
+}
+
+int bh_tree_build(bh_ctx* c, const double* pos, const double* mass, int64_t n, int32_t dim,
+                  int32_t leaf_size, bh_tree** out) {
+  BH_REQ(c && out, "NULL argument");
+  *out = nullptr;
+  BH_REQ(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  if (n <= 0) {
+    set_error("Cannot build tree with zero particles");
+    return BH_EEMPTY;
+  }
+  BH_REQ(pos && mass, "NULL positions or masses");
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* dp = s.get<double>((size_t)n * dim);
+  double* dm = s.get<double>((size_t)n);
+  if (!dp || !dm) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(dp, pos, (size_t)n * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  BH_HIP(hipMemcpyAsync(dm, mass, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return bh_tree_build_dev(c, dp, dm, n, dim, leaf_size, out);
+}
+
+void bh_tree_free(bh_tree* t) {
+  if (!t) return;
+  DeviceGuard g(t->ctx->device);
+  (void)hipStreamSynchronize(t->ctx->stream);
+  free_tree_arrays(t);
+  delete t;
+}
+
+int bh_tree_shape(const bh_tree* t, int64_t* n, int64_t* num_nodes, int32_t* dim, int32_t* leaf_size,
+                  int32_t* depth) {
+  BH_REQ(t, "tree is NULL");
+  if (n) *n = t->n;
+  if (num_nodes) *num_nodes = t->M;
+  if (dim) *dim = t->dim;
+  if (leaf_size) *leaf_size = t->leaf;
+  if (depth) *depth = t->depth;
+  return BH_OK;
+}
+
+int bh_tree_export(const bh_tree* t, double* center, double* half_size, double* mass, double* com,
+                   int64_t* first_child, int64_t* child_count, int64_t* start, int64_t* count,
+                   uint8_t* is_leaf, int64_t* perm) {
+  BH_REQ(t, "tree is NULL");
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  const size_t M = (size_t)t->M, n = (size_t)t->n, D = (size_t)t->dim;
+  if (center) BH_HIP(hipMemcpyAsync(center, t->center, M * D * 8, hipMemcpyDeviceToHost, c->stream));
+  if (half_size) BH_HIP(hipMemcpyAsync(half_size, t->half, M * 8, hipMemcpyDeviceToHost, c->stream));
+  if (mass) BH_HIP(hipMemcpyAsync(mass, t->mass, M * 8, hipMemcpyDeviceToHost, c->stream));
+  if (com) BH_HIP(hipMemcpyAsync(com, t->com, M * D * 8, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int32_t> fc, st, cn, pm;
+  if (first_child || child_count || is_leaf) {
+    fc.resize(M);
+    BH_HIP(hipMemcpyAsync(fc.data(), t->first_child, M * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (start) {
+    st.resize(M);
+    BH_HIP(hipMemcpyAsync(st.data(), t->start, M * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (count) {
+    cn.resize(M);
+    BH_HIP(hipMemcpyAsync(cn.data(), t->count, M * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (perm) {
+    pm.resize(n);
+    BH_HIP(hipMemcpyAsync(pm.data(), t->perm, n * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  BH_HIP(hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < M; k++) {
+    if (first_child) first_child[k] = fc[k];
+    if (child_count) child_count[k] = fc[k] >= 0 ? 2 : 0;
+    if (is_leaf) is_leaf[k] = fc[k] < 0;
+    if (start) start[k] = st[k];
+    if (count) count[k] = cn[k];
+  }
+  if (perm)
+    for (size_t i = 0; i < n; i++) perm[i] = pm[i];
+  return BH_OK;
+}
+
+int bh_tree_import(bh_ctx* c, int64_t n, int64_t M, int32_t dim, int32_t leaf_size,
+                   const double* center, const double* half_size, const double* mass,
+                   const double* com, const int64_t* first_child, const int64_t* child_count,
+                   const int64_t* start, const int64_t* count, const int64_t* perm,
+                   const double* src_pos, const double* src_mass, bh_tree** out) {
+  BH_REQ(c && out, "NULL argument");
+  *out = nullptr;
+  BH_REQ(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  BH_REQ(n > 0 && M > 0 && n < 0x7fffffffll && M < 0x7fffffffll, "bad tree shape n=%lld M=%lld",
+         (long long)n, (long long)M);
+  BH_REQ(center && half_size && mass && com && first_child && child_count && start && count && perm &&
+             src_pos && src_mass,
+         "NULL array");
+  // every index the kernels will follow is checked here
+  for (int64_t i = 0; i < n; i++) BH_REQ(perm[i] >= 0 && perm[i] < n, "perm[%lld] out of range", (long long)i);
+  std::vector<int32_t> skip(M, -1), par(M, -1), opento(M, -1);
+  std::vector<NodeD> nodes(M);
+  bool binary = true;
+  for (int64_t k = 0; k < M; k++) {
+    BH_REQ(start[k] >= 0 && count[k] >= 0 && start[k] + count[k] <= n, "node %lld range out of bounds",
+           (long long)k);
+    if (first_child[k] != -1) {
+      BH_REQ(first_child[k] > k && child_count[k] >= 0 && first_child[k] + child_count[k] <= M,
+             "node %lld children out of bounds", (long long)k);
+      if (child_count[k] != 2) binary = false;
+    }
+  }
+  for (int64_t k = 0; k < M; k++) {  // parents before children (first_child > k)
+    if (first_child[k] == -1) continue;
+    const int64_t fc = first_child[k], cc = child_count[k];
+    for (int64_t i = 0; i < cc; i++) {
+      skip[fc + i] = i > 0 ? (int32_t)(fc + i - 1) : skip[k];
+      par[fc + i] = (int32_t)k;
+    }
+    opento[k] = cc > 0 ? (int32_t)(fc + cc - 1) : skip[k];
+  }
+  std::vector<int32_t> leaves;
+  for (int64_t k = 0; k < M; k++) {
+    NodeD& r = nodes[k];
+    r.cx = com[k * dim];
+    r.cy = com[k * dim + 1];
+    r.cz = dim == 3 ? com[k * dim + 2] : 0.0;
+    r.m = mass[k];
+    r.size = 2 * half_size[k];
+    r.fc = (int32_t)first_child[k];
+    r.skip = skip[k];
+    r.start = (int32_t)start[k];
+    r.count = (int32_t)count[k];
+    r.pad[0] = opento[k];
+    r.pad[1] = 0;
+    if (first_child[k] == -1) leaves.push_back((int32_t)k);
+  }
+  std::sort(leaves.begin(), leaves.end(), [&](int32_t a, int32_t b) { return start[a] < start[b]; });
+  DeviceGuard g(c->device);
+  bh_tree* t = new bh_tree();
+  t->ctx = c;
+  t->n = n;
+  t->dim = dim;
+  t->leaf = leaf_size;
+  t->M = M;
+  t->nleaves = (int64_t)leaves.size();
+  t->depth = 0;
+  t->imported = true;
+  t->binary = binary;
+  int rc = BH_OK;
+  std::vector<int32_t> st(M), cn(M), fc(M), pm(n);
+  for (int64_t k = 0; k < M; k++) {
+    st[k] = (int32_t)start[k];
+    cn[k] = (int32_t)count[k];
+    fc[k] = (int32_t)first_child[k];
+  }
+  for (int64_t i = 0; i < n; i++) pm[i] = (int32_t)perm[i];
+  Scratch s(c);
+  double* dp = s.get<double>((size_t)n * dim);
+  double* dm = s.get<double>((size_t)n);
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (rc == BH_OK) {
+      hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) rc = hip_fail(e, "import copy", __FILE__, __LINE__);
+    }
+  };
+  if (!dp || !dm || !alloc_into(c, t->gbox, 2 * (size_t)dim) || !alloc_into(c, t->spart, (size_t)n) ||
+      !alloc_into(c, t->perm, (size_t)n) || alloc_topology(t) != BH_OK)
+    rc = BH_ENOMEM;
+  up(dp, src_pos, (size_t)n * dim * 8);
+  up(dm, src_mass, (size_t)n * 8);
+  up(t->perm, pm.data(), (size_t)n * 4);
+  up(t->start, st.data(), (size_t)M * 4);
+  up(t->count, cn.data(), (size_t)M * 4);
+  up(t->first_child, fc.data(), (size_t)M * 4);
+  up(t->parent, par.data(), (size_t)M * 4);
+  up(t->skip, skip.data(), (size_t)M * 4);
+  up(t->leaf_order, leaves.data(), leaves.size() * 4);
+  up(t->center, center, (size_t)M * dim * 8);
+  up(t->half, half_size, (size_t)M * 8);
+  up(t->mass, mass, (size_t)M * 8);
+  up(t->com, com, (size_t)M * dim * 8);
+  up(t->nodes, nodes.data(), (size_t)M * sizeof(NodeD));
+  if (rc == BH_OK) rc = bbox(c, dp, n, dim, t->gbox);
+  if (rc == BH_OK) rc = gather(c, dp, dm, reinterpret_cast<uint32_t*>(t->perm), n, dim, t->spart);
+  if (rc == BH_OK) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "import", __FILE__, __LINE__);
+  }
+  if (rc != BH_OK) {
+    free_tree_arrays(t);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return BH_OK;
+}
+
+int bh_tree_clone(const bh_tree* src, bh_tree** out) {
+  BH_REQ(src && out, "NULL argument");
+  *out = nullptr;
+  bh_ctx* c = src->ctx;
+  DeviceGuard g(c->device);
+  bh_tree* t = new bh_tree();
+  t->ctx = c;
+  t->n = src->n;
+  t->dim = src->dim;
+  t->leaf = src->leaf;
+  t->M = src->M;
+  t->nleaves = src->nleaves;
+  t->depth = src->depth;
+  t->imported = src->imported;
+  t->binary = src->binary;
+  int rc = BH_OK;
+  if (!alloc_into(c, t->gbox, 2 * (size_t)t->dim) || !alloc_into(c, t->spart, (size_t)t->n) ||
+      !alloc_into(c, t->perm, (size_t)t->n) || alloc_topology(t) != BH_OK)
+    rc = BH_ENOMEM;
+  auto cp = [&](void* dst, const void* s, size_t bytes) {
+    if (rc == BH_OK && bytes) {
+      hipError_t e = hipMemcpyAsync(dst, s, bytes, hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) rc = hip_fail(e, "clone copy", __FILE__, __LINE__);
+    }
+  };
+  const size_t M = (size_t)t->M, n = (size_t)t->n, D = (size_t)t->dim;
+  cp(t->gbox, src->gbox, 2 * D * 8);
+  cp(t->spart, src->spart, n * sizeof(double4));
+  cp(t->perm, src->perm, n * 4);
+  cp(t->start, src->start, M * 4);
+  cp(t->count, src->count, M * 4);
+  cp(t->first_child, src->first_child, M * 4);
+  cp(t->parent, src->parent, M * 4);
+  cp(t->skip, src->skip, M * 4);
+  cp(t->leaf_order, src->leaf_order, (size_t)t->nleaves * 4);
+  cp(t->center, src->center, M * D * 8);
+  cp(t->half, src->half, M * 8);
+  cp(t->mass, src->mass, M * 8);
+  cp(t->com, src->com, M * D * 8);
+  cp(t->bmin, src->bmin, M * D * 8);
+  cp(t->bmax, src->bmax, M * D * 8);
+  cp(t->nodes, src->nodes, M * sizeof(NodeD));
+  if (rc == BH_OK) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "clone", __FILE__, __LINE__);
+  }
+  if (rc != BH_OK) {
+    free_tree_arrays(t);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return BH_OK;
+}
+
+// ------------------------------------------------------------------ refit
+int bh_tree_should_refit_dev(bh_tree* t, const double* d_pos, int64_t n, int32_t dim,
+                             int32_t* refit_ok) {
+  BH_REQ(t && refit_ok, "NULL argument");
+  if (n != t->n || dim != t->dim) {
+    *refit_ok = 0;
+    return BH_OK;
+  }
+  DeviceGuard g(t->ctx->device);
+  int ok = 0;
+  BH_TRY(refit_check(t->ctx, t, d_pos, n, &ok));
+  *refit_ok = ok;
+  return BH_OK;
+}
+
+int bh_tree_should_refit(bh_tree* t, const double* pos, int64_t n, int32_t dim, int32_t* refit_ok) {
+  BH_REQ(t && refit_ok, "NULL argument");
+  if (n != t->n || dim != t->dim) {
+    *refit_ok = 0;
+    return BH_OK;
+  }
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* dp = s.get<double>((size_t)n * dim);
+  if (!dp) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(dp, pos, (size_t)n * dim * 8, hipMemcpyHostToDevice, c->stream));
+  return bh_tree_should_refit_dev(t, dp, n, dim, refit_ok);
+}
+
+int bh_tree_refit_dev(bh_tree* t, const double* d_pos, const double* d_mass) {
+  BH_REQ(t && d_pos, "NULL argument");
+  BH_REQ(t->binary, "refit needs a tree whose internal nodes have two children");
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  BH_TRY(gather(c, d_pos, d_mass, reinterpret_cast<uint32_t*>(t->perm), t->n, t->dim, t->spart));
+  BH_TRY(upward(c, t, 1));
+  BH_TRY(pack_nodes(c, t));
+  t->f32_ready = false;
+  return BH_OK;
+}
+
+int bh_tree_refit(bh_tree* t, const double* pos, const double* mass) {
+  BH_REQ(t && pos, "NULL argument");
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* dp = s.get<double>((size_t)t->n * t->dim);
+  double* dm = mass ? s.get<double>((size_t)t->n) : nullptr;
+  if (!dp || (mass && !dm)) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(dp, pos, (size_t)t->n * t->dim * 8, hipMemcpyHostToDevice, c->stream));
+  if (mass) BH_HIP(hipMemcpyAsync(dm, mass, (size_t)t->n * 8, hipMemcpyHostToDevice, c->stream));
+  BH_TRY(bh_tree_refit_dev(t, dp, dm));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+// ------------------------------------------------------------------ walks
+int bh_accel_self_dev(bh_tree* t, double theta, double eps, double G, double* d_out) {
+  BH_REQ(t && d_out, "NULL argument");
+  DeviceGuard g(t->ctx->device);
+  return walk_mode(t->ctx, t, WALK_ACC, nullptr, t->n, theta, eps, G, d_out, nullptr, nullptr,
+                   nullptr);
+}
+
+int bh_accel_self(bh_tree* t, double theta, double eps, double G, double* out) {
+  BH_REQ(t && out, "NULL argument");
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* d = s.get<double>((size_t)t->n * t->dim);
+  if (!d) return BH_ENOMEM;
+  BH_TRY(bh_accel_self_dev(t, theta, eps, G, d));
+  BH_HIP(hipMemcpyAsync(out, d, (size_t)t->n * t->dim * 8, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+int bh_accel_targets_dev(bh_tree* t, const double* d_tgt, int64_t m, double theta, double eps,
+                         double G, double* d_out) {
+  BH_REQ(t && (m == 0 || (d_tgt && d_out)), "NULL argument");
+  BH_REQ(m >= 0 && m < 0x7fffffffll, "bad target count");
+  DeviceGuard g(t->ctx->device);
+  return walk_mode(t->ctx, t, WALK_ACC, d_tgt, m, theta, eps, G, d_out, nullptr, nullptr, nullptr);
+}
+
+int bh_accel_targets(bh_tree* t, const double* tgt, int64_t m, double theta, double eps, double G,
+                     double* out) {
+  BH_REQ(t && (m == 0 || (tgt && out)), "NULL argument");
+  if (m == 0) return BH_OK;
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* dt = s.get<double>((size_t)m * t->dim);
+  double* d = s.get<double>((size_t)m * t->dim);
+  if (!dt || !d) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(dt, tgt, (size_t)m * t->dim * 8, hipMemcpyHostToDevice, c->stream));
+  BH_TRY(bh_accel_targets_dev(t, dt, m, theta, eps, G, d));
+  BH_HIP(hipMemcpyAsync(out, d, (size_t)m * t->dim * 8, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+int bh_interaction_lists(bh_tree* t, const double* tgt, int64_t m, double theta, int64_t* ptr,
+                         int32_t* ids, int64_t cap, int64_t* total) {
+  BH_REQ(t && ptr, "NULL argument");
+  BH_REQ(tgt || m == t->n, "self mode needs m == n");
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  const double* dt = nullptr;
+  if (tgt && m > 0) {
+    double* d = s.get<double>((size_t)m * t->dim);
+    if (!d) return BH_ENOMEM;
+    BH_HIP(hipMemcpyAsync(d, tgt, (size_t)m * t->dim * 8, hipMemcpyHostToDevice, c->stream));
+    dt = d;
+  }
+  int64_t* dcnt = s.get<int64_t>((size_t)m + 1);
+  if (!dcnt) return BH_ENOMEM;
+  BH_TRY(walk_mode(c, t, WALK_LCOUNT, dt, m, theta, 0.0, 1.0, nullptr, nullptr, nullptr, dcnt));
+  std::vector<int64_t> cnt(m);
+  if (m) BH_HIP(hipMemcpyAsync(cnt.data(), dcnt, m * 8, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  ptr[0] = 0;
+  for (int64_t i = 0; i < m; i++) ptr[i + 1] = ptr[i] + cnt[i];
+  const int64_t tot = ptr[m];
+  if (total) *total = tot;
+  if (!ids || cap <= 0 || tot == 0) return BH_OK;
+  int64_t* dptr = s.get<int64_t>((size_t)m + 1);
+  int32_t* dids = s.get<int32_t>((size_t)tot);
+  if (!dptr || !dids) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(dptr, ptr, (m + 1) * 8, hipMemcpyHostToDevice, c->stream));
+  BH_TRY(walk_mode(c, t, WALK_LFILL, dt, m, theta, 0.0, 1.0, nullptr, dptr, dids, nullptr));
+  BH_HIP(hipMemcpyAsync(ids, dids, (size_t)std::min(tot, cap) * 4, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+int bh_counters(bh_tree* t, const double* tgt, int64_t m, double theta, int64_t* out) {
+  BH_REQ(t && out, "NULL argument");
+  BH_REQ(tgt || m == t->n, "self mode needs m == n");
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  const double* dt = nullptr;
+  if (tgt && m > 0) {
+    double* d = s.get<double>((size_t)m * t->dim);
+    if (!d) return BH_ENOMEM;
+    BH_HIP(hipMemcpyAsync(d, tgt, (size_t)m * t->dim * 8, hipMemcpyHostToDevice, c->stream));
+    dt = d;
+  }
+  int64_t* dout = s.get<int64_t>((size_t)m * 5);
+  if (!dout) return BH_ENOMEM;
+  BH_TRY(walk_mode(c, t, WALK_STATS, dt, m, theta, 0.0, 1.0, nullptr, nullptr, nullptr, dout));
+  if (m) BH_HIP(hipMemcpyAsync(out, dout, (size_t)m * 5 * 8, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+// ------------------------------------------------------------------ FP32 form
+int bh_accel_f32_self_dev(bh_tree* t, double theta, double eps, double G, const bh_f32_opts* o,
+                          double* d_out, int64_t* stats) {
+  BH_REQ(t && d_out, "NULL argument");
+  DeviceGuard g(t->ctx->device);
+  return walk_f32(t->ctx, t, nullptr, t->n, theta, eps, G, o, d_out, stats);
+}
+
+int bh_accel_f32_self(bh_tree* t, double theta, double eps, double G, const bh_f32_opts* o,
+                      double* out, int64_t* stats) {
+  BH_REQ(t && out, "NULL argument");
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* d = s.get<double>((size_t)t->n * t->dim);
+  if (!d) return BH_ENOMEM;
+  BH_TRY(bh_accel_f32_self_dev(t, theta, eps, G, o, d, stats));
+  BH_HIP(hipMemcpyAsync(out, d, (size_t)t->n * t->dim * 8, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+int bh_accel_f32_targets_dev(bh_tree* t, const double* d_tgt, int64_t m, double theta, double eps,
+                             double G, const bh_f32_opts* o, double* d_out, int64_t* stats) {
+  BH_REQ(t && (m == 0 || (d_tgt && d_out)), "NULL argument");
+  BH_REQ(m >= 0 && m < 0x7fffffffll, "bad target count");
+  DeviceGuard g(t->ctx->device);
+  return walk_f32(t->ctx, t, d_tgt, m, theta, eps, G, o, d_out, stats);
+}
+
+int bh_accel_f32_targets(bh_tree* t, const double* tgt, int64_t m, double theta, double eps,
+                         double G, const bh_f32_opts* o, double* out, int64_t* stats) {
+  BH_REQ(t && (m == 0 || (tgt && out)), "NULL argument");
+  if (m == 0) return BH_OK;
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* dt = s.get<double>((size_t)m * t->dim);
+  double* d = s.get<double>((size_t)m * t->dim);
+  if (!dt || !d) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(dt, tgt, (size_t)m * t->dim * 8, hipMemcpyHostToDevice, c->stream));
+  BH_TRY(bh_accel_f32_targets_dev(t, dt, m, theta, eps, G, o, d, stats));
+  BH_HIP(hipMemcpyAsync(out, d, (size_t)m * t->dim * 8, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+// ------------------------------------------------------------------ direct sum
+int bh_direct_dev(bh_ctx* c, const double* d_src, const double* d_mass, int64_t n,
+                  const double* d_tgt, int64_t m, int32_t dim, int32_t self_mode, double eps,
+                  double G, double* d_acc, double* d_pot) {
+  BH_REQ(c, "ctx is NULL");
+  BH_REQ(dim == 2 || dim == 3, "dim must be 2 or 3");
+  BH_REQ(!self_mode || m == n, "self mode needs m == n");
+  BH_REQ(n >= 0 && m >= 0, "negative size");
+  DeviceGuard g(c->device);
+  return direct(c, d_src, d_mass, n, self_mode ? d_src : d_tgt, m, dim, self_mode, eps, G, d_acc,
+                d_pot);
+}
+
+int bh_direct(bh_ctx* c, const double* src, const double* mass, int64_t n, const double* tgt,
+              int64_t m, int32_t dim, int32_t self_mode, double eps, double G, double* acc,
+              double* pot) {
+  BH_REQ(c, "ctx is NULL");
+  BH_REQ(dim == 2 || dim == 3, "dim must be 2 or 3");
+  BH_REQ(!self_mode || m == n, "self mode needs m == n");
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* ds = s.get<double>((size_t)n * dim);
+  double* dm = s.get<double>((size_t)n);
+  double* dt = self_mode ? ds : s.get<double>((size_t)m * dim);
+  double* da = acc ? s.get<double>((size_t)m * dim) : nullptr;
+  double* dp = pot ? s.get<double>((size_t)m) : nullptr;
+  if (!ds || !dm || !dt || (acc && !da) || (pot && !dp)) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(ds, src, (size_t)n * dim * 8, hipMemcpyHostToDevice, c->stream));
+  BH_HIP(hipMemcpyAsync(dm, mass, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+  if (!self_mode) BH_HIP(hipMemcpyAsync(dt, tgt, (size_t)m * dim * 8, hipMemcpyHostToDevice, c->stream));
+  BH_TRY(direct(c, ds, dm, n, dt, m, dim, self_mode, eps, G, da, dp));
+  if (acc) BH_HIP(hipMemcpyAsync(acc, da, (size_t)m * dim * 8, hipMemcpyDeviceToHost, c->stream));
+  if (pot) BH_HIP(hipMemcpyAsync(pot, dp, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+}  // extern "C"

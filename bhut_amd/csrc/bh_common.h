@@ -1,0 +1,182 @@
+// bh_common.h -- shared types and device helpers of libbhut_hip.
+//
+// Written for gfx950 (MI355X, wave64).  Every translation unit is compiled
+// with -ffp-contract=off: the plain FP64 form reproduces bhut's rounding
+// sequence operation by operation, and the FP32 form spells out its fused
+// multiply-adds with fmaf().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstddef>
+#include <mutex>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bhut_hip.h"
+
+namespace bh {
+
+constexpr int WAVE = 64;
+constexpr int MAX_DEPTH = 64;  // topology tables; depth <= log2(N) + 2
+
+// ------------------------------------------------------------ errors
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define BH_HIP(call)                                                       \
+  do {                                                                     \
+    hipError_t _e = (call);                                                \
+    if (_e != hipSuccess) return ::bh::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define BH_LAUNCH_CHECK() BH_HIP(hipGetLastError())
+
+// ------------------------------------------------------------ records
+// Traversal record of one node (64 B, one cache line).  `skip` is the next
+// node of bhut's pop order once this node's subtree is done; the LIFO stack
+// walk of bh.py:506-546 pops the right child first, so the pop order is the
+// right-first preorder and `skip` reproduces it without a stack.
+struct __attribute__((aligned(64))) NodeD {
+  double cx, cy, cz, m;  // centre of mass, mass
+  double size;           // 2*half_size (bh.py:589)
+  int32_t fc;            // first child (-1 for leaves)
+  int32_t skip;          // next node after this subtree (-1: walk ends)
+  int32_t start, count;  // particle range in sorted order
+  int32_t pad[2];
+};
+static_assert(sizeof(NodeD) == 64, "NodeD layout");
+
+// ------------------------------------------------------------ context
+struct Ctx;
+
+struct Topo {
+  int64_t n = 0, leaf = 0, M = 0, nleaves = 0;
+  int depth = 0;  // maximum node depth (root = 0)
+};
+
+}  // namespace bh
+
+struct bh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;      // stream all work is enqueued on
+  hipStream_t own_stream = nullptr;  // the context's own (default-priority) stream
+  int num_cus = 0;
+  // caching device allocator: exact-size free lists
+  std::mutex mu;
+  std::multimap<size_t, void*> free_blocks;
+  std::unordered_map<void*, size_t> live;
+  size_t cached_bytes = 0;
+  // small pinned staging buffer for flags / counters read back by the host
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  // FP32-form list capacities remembered between calls (entries per call)
+  int64_t f32_mono_cap = 0, f32_leaf_cap = 0;
+};
+
+struct bh_tree {
+  bh_ctx* ctx = nullptr;
+  int64_t n = 0;
+  int dim = 3;
+  int leaf = 32;
+  int64_t M = 0;
+  int64_t nleaves = 0;
+  int depth = 0;
+  bool imported = false;  // built from host TreeData arrays
+  bool binary = true;     // every internal node has two children (refit needs it)
+  // particles in Morton order
+  double4* spart = nullptr;   // n: x, y, z (0 in 2-D), m
+  int32_t* perm = nullptr;    // n: sorted index -> original index
+  // topology (pure function of n, leaf)
+  int32_t* start = nullptr;
+  int32_t* count = nullptr;
+  int32_t* first_child = nullptr;
+  int32_t* parent = nullptr;
+  int32_t* skip = nullptr;
+  int32_t* leaf_order = nullptr;  // leaves left to right
+  // geometry and monopoles in TreeData layout
+  double* center = nullptr;   // M*dim
+  double* half = nullptr;     // M
+  double* mass = nullptr;     // M
+  double* com = nullptr;      // M*dim
+  double* bmin = nullptr;     // M*dim node box (build only)
+  double* bmax = nullptr;
+  double* gbox = nullptr;     // 2*dim global pmin/pmax
+  bh::NodeD* nodes = nullptr; // packed traversal records
+  // FP32 form: particles relative to their leaf centre
+  float4* spart32 = nullptr;
+  bool f32_ready = false;
+};
+
+namespace bh {
+
+// ------------------------------------------------------------ memory
+void* dalloc(bh_ctx* c, size_t bytes);
+void dfree(bh_ctx* c, void* p);
+template <class T>
+T* dalloc_t(bh_ctx* c, size_t n) {
+  return static_cast<T*>(dalloc(c, n * sizeof(T) + (n == 0 ? 16 : 0)));
+}
+
+// Scoped scratch allocations returned to the cache on exit.
+struct Scratch {
+  bh_ctx* c;
+  std::vector<void*> ps;
+  explicit Scratch(bh_ctx* ctx) : c(ctx) {}
+  template <class T>
+  T* get(size_t n) {
+    T* p = dalloc_t<T>(c, n);
+    if (p) ps.push_back(p);
+    return p;
+  }
+  ~Scratch() {
+    for (void* p : ps) dfree(c, p);
+  }
+};
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ------------------------------------------------------------ host entry points per TU
+// bh_sort.hip
+int bbox(bh_ctx* c, const double* pos, int64_t n, int dim, double* d_box);
+int morton_iota(bh_ctx* c, const double* pos, int64_t n, int dim, const double* d_box, int clamp,
+                uint64_t* keys, uint32_t* vals);
+int radix_sort(bh_ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int bits, uint64_t** keys_out,
+               uint32_t** vals_out, uint64_t* keys_alt, uint32_t* vals_alt);
+int iota(bh_ctx* c, uint32_t* v, int64_t n);
+int gather(bh_ctx* c, const double* pos, const double* mass, const uint32_t* perm, int64_t n,
+           int dim, double4* spart);
+// bh_build.hip
+int topology_plan(int64_t N, int64_t L, int64_t* M, int64_t* nleaves, int* depth);
+int topology(bh_ctx* c, bh_tree* t);
+int upward(bh_ctx* c, bh_tree* t, int refit);
+int pack_nodes(bh_ctx* c, bh_tree* t);
+int refit_check(bh_ctx* c, const bh_tree* t, const double* pos, int64_t n, int* ok);
+int fill_u32(bh_ctx* c, uint32_t* p, uint32_t v, int64_t n);
+// bh_walk.hip
+enum { WALK_ACC = 0, WALK_LCOUNT = 1, WALK_LFILL = 2, WALK_STATS = 3 };
+int order_targets(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, uint32_t* torder);
+int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t m, double theta,
+              double eps, double G, double* out, const int64_t* lptr, int32_t* lids, int64_t* stats);
+// bh_group.hip
+int f32_prepare(bh_ctx* c, bh_tree* t);
+int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, double eps,
+             double G, const bh_f32_opts* o, double* out, int64_t* stats);
+// bh_direct.hip
+int direct(bh_ctx* c, const double* src, const double* mass, int64_t n, const double* tgt,
+           int64_t m, int dim, int self_mode, double eps, double G, double* acc, double* pot);
+
+// ------------------------------------------------------------ device helpers
+// numpy.minimum / numpy.maximum: a NaN in either operand wins.
+__device__ __forceinline__ double np_min(double a, double b) { return (a < b || a != a) ? a : b; }
+__device__ __forceinline__ double np_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+}  // namespace bh

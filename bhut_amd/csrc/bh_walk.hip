@@ -1,0 +1,235 @@
+// bh_walk.hip -- the plain FP64 per-target walk (bit-faithful form).
+//
+// Reference: _barnes_hut_single_target (bhut/traverse/bh.py:485-548) and
+// _barnes_hut_single_target_external (bh.py:83-146), _should_open_node
+// (bh.py:576-601), acc_monopole_node/acc_monopole (bhut/traverse/kernels.py:
+// 14-69,136-168), the Numba leaf loops (bh.py:174-280) with their pure-Python
+// fallback (bh.py:307-311,339-343,384-425).
+//
+// One lane per target, targets in Morton order so a wavefront walks similar
+// paths.  The reference's LIFO stack pops the right child first, i.e. visits
+// nodes in right-first preorder; each record carries the node that follows
+// its subtree in that order (`skip`) and the child popped first when it is
+// opened (`pad[0]` = first_child + 1), so the walk needs no stack and visits,
+// accepts and sums in exactly the reference's order.
+#include "bh_common.h"
+
+namespace bh {
+
+template <int DIM, bool SELF, int MODE>
+__global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
+                                              const double4* __restrict__ spart,
+                                              const int32_t* __restrict__ perm, int64_t m,
+                                              const double* __restrict__ tgt,
+                                              const uint32_t* __restrict__ torder, double theta,
+                                              double eps2, double G, double* __restrict__ out,
+                                              const int64_t* __restrict__ lptr,
+                                              int32_t* __restrict__ lids,
+                                              int64_t* __restrict__ stats) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  int64_t orig;
+  double tx, ty, tz = 0.0;
+  if (SELF) {
+    orig = perm[t];
+    const double4 p = spart[t];
+    tx = p.x;
+    ty = p.y;
+    tz = p.z;
+  } else {
+    orig = torder[t];
+    tx = tgt[orig * DIM + 0];
+    ty = tgt[orig * DIM + 1];
+    if (DIM == 3) tz = tgt[orig * DIM + 2];
+  }
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  int64_t nl = 0, wpos = 0;
+  int64_t c_pop = 0, c_test = 0, c_mono = 0, c_leaf = 0, c_pair = 0;
+  if (MODE == WALK_LFILL) wpos = lptr[orig];
+  int v = 0;
+  while (v >= 0) {
+    const NodeD nd = nodes[v];
+    if (MODE == WALK_STATS) c_pop++;
+    if (nd.m == 0.0) {  // bh.py:512-513
+      v = nd.skip;
+      continue;
+    }
+    if (nd.fc < 0) {  // leaf: direct particle-particle sum (bh.py:518-522)
+      if (MODE == WALK_LCOUNT) nl++;
+      if (MODE == WALK_LFILL) lids[wpos++] = v;
+      if (MODE == WALK_STATS) {
+        c_leaf++;
+        c_pair += nd.count;
+        if (SELF && t >= nd.start && t < nd.start + nd.count) c_pair--;
+      }
+      if (MODE == WALK_ACC) {
+        const int64_t j0 = nd.start, j1 = nd.start + nd.count;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        bool zdiv = false;
+        // Numba loop (bh.py:249-278): r = sqrt(r2 + eps^2); 1/r; r^-3
+        for (int64_t j = j0; j < j1; j++) {
+          if (SELF && j == t) continue;
+          const double4 q = spart[j];
+          const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+          double rsq = dx * dx;
+          rsq = rsq + dy * dy;
+          if (DIM == 3) rsq = rsq + dz * dz;
+          rsq = rsq + eps2;
+          const double r = sqrt(rsq);
+          if (r == 0.0) {  // Numba raises ZeroDivisionError here
+            zdiv = true;
+            break;
+          }
+          const double rinv = 1.0 / r;
+          const double ric = rinv * rinv * rinv;
+          const double mag = G * q.w * ric;
+          px = px + mag * dx;
+          py = py + mag * dy;
+          if (DIM == 3) pz = pz + mag * dz;
+        }
+        if (zdiv) {
+          // the whole leaf again with the pure-Python loop (bh.py:384-425):
+          // G m / (r2 + eps^2)^1.5, zero where r2 + eps^2 == 0
+          px = py = pz = 0.0;
+          for (int64_t j = j0; j < j1; j++) {
+            if (SELF && j == t) continue;
+            const double4 q = spart[j];
+            const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+            double r2 = dx * dx;
+            r2 = r2 + dy * dy;
+            if (DIM == 3) r2 = r2 + dz * dz;
+            const double r2s = r2 + eps2;
+            if (r2s == 0.0) {
+              px = px + 0.0;
+              py = py + 0.0;
+              pz = pz + 0.0;
+              continue;
+            }
+            const double f = (G * q.w) / (r2s * sqrt(r2s));
+            px = px + f * dx;
+            py = py + f * dy;
+            if (DIM == 3) pz = pz + f * dz;
+          }
+        }
+        ax = ax + px;
+        ay = ay + py;
+        if (DIM == 3) az = az + pz;
+      }
+      v = nd.skip;
+      continue;
+    }
+    // opening test (bh.py:586-601): d = ||t - com|| as OpenBLAS ddot forms
+    // it (x*x, then fma for y and z), open when d == 0 or size/d >= theta
+    if (MODE == WALK_STATS) c_test++;
+    const double ux = tx - nd.cx, uy = ty - nd.cy, uz = tz - nd.cz;
+    double d2 = ux * ux;
+    d2 = fma(uy, uy, d2);
+    if (DIM == 3) d2 = fma(uz, uz, d2);
+    const double dist = sqrt(d2);
+    const bool open = (dist == 0.0) || (nd.size / dist >= theta);
+    if (open) {
+      v = nd.pad[0];
+      continue;
+    }
+    // monopole (kernels.py:57-69 via acc_monopole_node)
+    if (MODE == WALK_LCOUNT) nl++;
+    if (MODE == WALK_LFILL) lids[wpos++] = v;
+    if (MODE == WALK_STATS) c_mono++;
+    if (MODE == WALK_ACC) {
+      const double dx = nd.cx - tx, dy = nd.cy - ty, dz = nd.cz - tz;
+      double r2 = dx * dx;
+      r2 = r2 + dy * dy;
+      if (DIM == 3) r2 = r2 + dz * dz;
+      const double r2s = r2 + eps2;
+      if (r2s != 0.0) {
+        const double f = (G * nd.m) / (r2s * sqrt(r2s));
+        ax = ax + f * dx;
+        ay = ay + f * dy;
+        if (DIM == 3) az = az + f * dz;
+      } else {
+        ax = ax + 0.0;
+        ay = ay + 0.0;
+        az = az + 0.0;
+      }
+    }
+    v = nd.skip;
+  }
+  if (MODE == WALK_ACC) {
+    out[orig * DIM + 0] = ax;
+    out[orig * DIM + 1] = ay;
+    if (DIM == 3) out[orig * DIM + 2] = az;
+  }
+  if (MODE == WALK_LCOUNT) stats[orig] = nl;
+  if (MODE == WALK_STATS) {
+    int64_t* o = stats + orig * 5;
+    o[0] = c_pop;
+    o[1] = c_test;
+    o[2] = c_mono;
+    o[3] = c_leaf;
+    o[4] = c_pair;
+  }
+}
+
+template <int DIM, bool SELF>
+static void launch_walk(bh_ctx* c, int mode, const bh_tree* t, int64_t m, const double* tgt,
+                        const uint32_t* torder, double theta, double eps2, double G, double* out,
+                        const int64_t* lptr, int32_t* lids, int64_t* stats) {
+  const dim3 grid((unsigned)cdiv(m, 256)), block(256);
+#define BH_W(MODE)                                                                             \
+  hipLaunchKernelGGL((k_walk<DIM, SELF, MODE>), grid, block, 0, c->stream, t->nodes, t->spart,  \
+                     t->perm, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats)
+  switch (mode) {
+    case WALK_ACC: BH_W(WALK_ACC); break;
+    case WALK_LCOUNT: BH_W(WALK_LCOUNT); break;
+    case WALK_LFILL: BH_W(WALK_LFILL); break;
+    default: BH_W(WALK_STATS); break;
+  }
+#undef BH_W
+}
+
+// Orders external targets along the tree's Morton curve (codes relative to
+// the source box, clamped) so each wavefront walks nearby targets.
+int order_targets(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, uint32_t* torder) {
+  if (m == 0) return BH_OK;
+  Scratch s(c);
+  uint64_t* k0 = s.get<uint64_t>(m);
+  uint64_t* k1 = s.get<uint64_t>(m);
+  uint32_t* v1 = s.get<uint32_t>(m);
+  if (!k0 || !k1 || !v1) return BH_ENOMEM;
+  int rc = morton_iota(c, tgt, m, t->dim, t->gbox, 1, k0, torder);
+  if (rc) return rc;
+  uint64_t* ko;
+  uint32_t* vo;
+  rc = radix_sort(c, k0, torder, m, 21 * t->dim, &ko, &vo, k1, v1);
+  if (rc) return rc;
+  if (vo != torder) BH_HIP(hipMemcpyAsync(torder, vo, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+  return BH_OK;
+}
+
+// Runs one walk mode.  tgt == nullptr: self mode (m == n).  For list fill,
+// lptr holds per-target offsets (original order); for list count and for
+// counters, `stats` receives per-target values.
+int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t m, double theta,
+              double eps, double G, double* out, const int64_t* lptr, int32_t* lids, int64_t* stats) {
+  if (m == 0) return BH_OK;
+  const double eps2 = eps * eps;
+  Scratch s(c);
+  uint32_t* torder = nullptr;
+  if (tgt) {
+    torder = s.get<uint32_t>(m);
+    if (!torder) return BH_ENOMEM;
+    int rc = order_targets(c, t, tgt, m, torder);
+    if (rc) return rc;
+  }
+  if (t->dim == 3) {
+    if (tgt) launch_walk<3, false>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+    else launch_walk<3, true>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+  } else {
+    if (tgt) launch_walk<2, false>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+    else launch_walk<2, true>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+  }
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+}  // namespace bh

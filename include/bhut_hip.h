@@ -1,0 +1,149 @@
+/*
+ * bhut_hip.h -- C ABI of libbhut_hip.so, the MI355X (gfx950) implementation
+ * of bhut's Barnes-Hut force path.
+ *
+ * Plain pointers and sizes only.  Functions without a suffix take HOST
+ * pointers and are synchronous; `_dev` variants take DEVICE pointers (for
+ * example torch.Tensor.data_ptr() of tensors on the same device), enqueue on
+ * the context stream and return without synchronising, except where noted.
+ *
+ * Status: 0 on success, a negative BH_E* code on failure; bh_last_error()
+ * returns the message of the calling thread's last failure.
+ *
+ * Reference interfaces are cited as path:line in maroba/bhut.  The reference
+ * has no C ABI of its own; every entry point below replaces a Python function
+ * on the NumPy path, and bhut_amd/_native.py binds them with ctypes.
+ */
+#ifndef BHUT_HIP_H
+#define BHUT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_OK 0
+#define BH_EINVAL (-1)    /* invalid argument (maps to ValueError)          */
+#define BH_EHIP (-2)      /* HIP runtime error                              */
+#define BH_ENOMEM (-3)    /* device allocation failed                       */
+#define BH_EEMPTY (-4)    /* zero particles (build.py:81-82)                */
+#define BH_ENODEV (-5)    /* no HIP device                                  */
+#define BH_EOVERFLOW (-6) /* an internal capacity was exceeded              */
+
+typedef struct bh_ctx bh_ctx;   /* device, stream, cached allocations */
+typedef struct bh_tree bh_tree; /* a tree resident in device memory   */
+
+/* Options of the FP32 relative-coordinate, compensated, grouped form. */
+typedef struct {
+  int32_t group_size; /* targets per group: 64 (one per lane) or 128 (two)  */
+  int32_t kahan;      /* 1: Kahan-compensated per-target sums               */
+  int32_t stack_cap;  /* LDS stack entries of the group walk (<= 4096)      */
+  int32_t reserved;
+} bh_f32_opts;
+
+/* ---------------------------------------------------------------- context */
+const char *bh_last_error(void);
+int bh_version(void);
+int bh_device_count(void);
+int bh_ctx_create(int device, bh_ctx **out);
+void bh_ctx_destroy(bh_ctx *ctx);
+int bh_ctx_synchronize(bh_ctx *ctx);
+void *bh_ctx_stream(bh_ctx *ctx);             /* the hipStream_t used        */
+/* Enqueue all further work on `stream` (a hipStream_t, e.g. torch's current
+ * stream); NULL returns to the context's own stream.  Synchronises first. */
+int bh_ctx_set_stream(bh_ctx *ctx, void *stream);
+int bh_ctx_trim(bh_ctx *ctx);                 /* release cached allocations  */
+
+/* ------------------------------------------------------------------ build */
+/* build_tree (bhut/tree/build.py:19-107, via bhut/api.py:155-191): bounding
+ * box, Morton codes, stable sort, median-split topology, tight node boxes,
+ * mass and centre of mass.  pos is (n, dim) row-major float64. */
+int bh_tree_build(bh_ctx *ctx, const double *pos, const double *mass, int64_t n, int32_t dim,
+                  int32_t leaf_size, bh_tree **out);
+int bh_tree_build_dev(bh_ctx *ctx, const double *d_pos, const double *d_mass, int64_t n,
+                      int32_t dim, int32_t leaf_size, bh_tree **out); /* synchronises */
+/* TreeData (bhut/tree/node.py:15-77) in host memory plus the source
+ * positions/masses the traversal will use (bh.py:428 `positions, masses`). */
+int bh_tree_import(bh_ctx *ctx, int64_t n, int64_t num_nodes, int32_t dim, int32_t leaf_size,
+                   const double *center, const double *half_size, const double *mass,
+                   const double *com, const int64_t *first_child, const int64_t *child_count,
+                   const int64_t *start, const int64_t *count, const int64_t *perm,
+                   const double *src_pos, const double *src_mass, bh_tree **out);
+int bh_tree_clone(const bh_tree *t, bh_tree **out);
+void bh_tree_free(bh_tree *t);
+int bh_tree_shape(const bh_tree *t, int64_t *n, int64_t *num_nodes, int32_t *dim,
+                  int32_t *leaf_size, int32_t *depth);
+/* TreeData arrays (node.py:59-77) copied to host buffers; any may be NULL. */
+int bh_tree_export(const bh_tree *t, double *center, double *half_size, double *mass, double *com,
+                   int64_t *first_child, int64_t *child_count, int64_t *start, int64_t *count,
+                   uint8_t *is_leaf, int64_t *perm);
+
+/* ------------------------------------------------------------------ refit */
+/* should_refit_vs_rebuild (bhut/tree/refit.py:195-251): *refit_ok = 1 when
+ * every particle lies within 1.1 root half-sizes of the stored root centre. */
+int bh_tree_should_refit(bh_tree *t, const double *pos, int64_t n, int32_t dim, int32_t *refit_ok);
+int bh_tree_should_refit_dev(bh_tree *t, const double *d_pos, int64_t n, int32_t dim,
+                             int32_t *refit_ok); /* synchronises */
+/* refit_tree (refit.py:17-109): new mass and centre of mass from positions in
+ * ORIGINAL order; centres and half sizes stay as built.  mass may be NULL to
+ * keep the current masses. */
+int bh_tree_refit(bh_tree *t, const double *pos, const double *mass);
+int bh_tree_refit_dev(bh_tree *t, const double *d_pos, const double *d_mass);
+
+/* ---------------------------------------------------- accelerations (FP64) */
+/* barnes_hut_accelerations (bhut/traverse/bh.py:428-482; evaluate path of
+ * api.py:367-429 and Tree.evaluate_accelerations api.py:733-786): out is
+ * (n, dim) in the original particle order; source i skips target i. */
+int bh_accel_self(bh_tree *t, double theta, double softening, double G, double *out);
+int bh_accel_self_dev(bh_tree *t, double theta, double softening, double G, double *d_out);
+/* barnes_hut_accelerations_targets (bh.py:30-80; Tree.accelerations with
+ * targets, api.py:608-688): no self exclusion. */
+int bh_accel_targets(bh_tree *t, const double *targets, int64_t m, double theta, double softening,
+                     double G, double *out);
+int bh_accel_targets_dev(bh_tree *t, const double *d_targets, int64_t m, double theta,
+                         double softening, double G, double *d_out);
+
+/* ------------------------------------------- accelerations (FP32, chosen) */
+/* Same contract as bh_accel_self / bh_accel_targets, computed in FP32 on
+ * coordinates relative to leaf and group centres, with a conservative group
+ * opening test; opts may be NULL for defaults.  stats (may be NULL) receives
+ * 4 int64: monopole interactions, particle pairs, groups, list entries. */
+int bh_accel_f32_self(bh_tree *t, double theta, double softening, double G,
+                      const bh_f32_opts *opts, double *out, int64_t *stats);
+int bh_accel_f32_self_dev(bh_tree *t, double theta, double softening, double G,
+                          const bh_f32_opts *opts, double *d_out, int64_t *stats);
+int bh_accel_f32_targets(bh_tree *t, const double *targets, int64_t m, double theta,
+                         double softening, double G, const bh_f32_opts *opts, double *out,
+                         int64_t *stats);
+int bh_accel_f32_targets_dev(bh_tree *t, const double *d_targets, int64_t m, double theta,
+                             double softening, double G, const bh_f32_opts *opts, double *d_out,
+                             int64_t *stats);
+
+/* ------------------------------------------------- lists and statistics */
+/* Per-target interaction lists of the walk in _barnes_hut_single_target
+ * (bh.py:485-548): node ids in pop order, leaves and accepted monopoles.
+ * targets == NULL selects self mode (m must equal n).  ptr has m+1 entries
+ * (original target order); ids receives min(total, cap) entries. */
+int bh_interaction_lists(bh_tree *t, const double *targets, int64_t m, double theta,
+                         int64_t *ptr, int32_t *ids, int64_t cap, int64_t *total);
+/* Exact per-target counters replacing compute_tree_statistics (bh.py:604-700):
+ * out is (m, 5) int64: nodes visited, opening tests, monopoles, leaves,
+ * particle pairs (self excluded).  targets == NULL selects self mode. */
+int bh_counters(bh_tree *t, const double *targets, int64_t m, double theta, int64_t *out);
+
+/* --------------------------------------------------------- direct sums */
+/* FP64 direct summation, the theta = 0 limit of the walk (bh.py:600-601):
+ * acc (m, dim) and/or pot (m) (Plummer potential -G m / sqrt(r^2 + eps^2)),
+ * either may be NULL.  self_mode skips j == i (m must equal n). */
+int bh_direct(bh_ctx *ctx, const double *src, const double *mass, int64_t n, const double *targets,
+              int64_t m, int32_t dim, int32_t self_mode, double softening, double G, double *acc,
+              double *pot);
+int bh_direct_dev(bh_ctx *ctx, const double *d_src, const double *d_mass, int64_t n,
+                  const double *d_targets, int64_t m, int32_t dim, int32_t self_mode,
+                  double softening, double G, double *d_acc, double *d_pot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BHUT_HIP_H */

@@ -1,0 +1,251 @@
+"""Parity of the HIP plain form with the oracle and the golden fixtures.
+
+Bit-exact: Morton order (perm), topology, node boxes, mass, centre of mass,
+refit, per-target interaction lists and exact counters.  Accelerations: the
+same summation order as bhut; the only arithmetic difference is r2s*sqrt(r2s)
+in place of libm pow(r2s, 1.5) in monopoles, so per target
+    ||a_gpu - a_ref|| <= 1e-12 * ||a_ref||   (with a floor of 1e-12 * max||a_ref||
+for targets whose acceleration cancels to near zero).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden_files
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TREE_FIELDS = ["perm", "start", "count", "first_child", "child_count", "is_leaf", "center", "half_size", "mass",
+               "com"]
+
+
+def bits_equal(a, b, name=""):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, name
+    if a.dtype == np.float64:
+        np.testing.assert_array_equal(a.view(np.uint64), np.asarray(b, np.float64).view(np.uint64), err_msg=name)
+    else:
+        np.testing.assert_array_equal(a, b, err_msg=name)
+
+
+def acc_close(got, ref, tol=1e-12):
+    got, ref = np.asarray(got), np.asarray(ref)
+    assert got.shape == ref.shape
+    nr = np.linalg.norm(ref, axis=1)
+    floor = tol * (nr.max() if len(nr) else 0.0)
+    err = np.linalg.norm(got - ref, axis=1)
+    bad = err > np.maximum(tol * nr, floor)
+    assert not bad.any(), f"{bad.sum()} targets off; worst rel {np.max(err / np.maximum(nr, 1e-300)):.3e}"
+
+
+@pytest.fixture(scope="module")
+def native():
+    from bhut_amd import _native
+    if _native.device_count() < 1:
+        pytest.skip("no HIP device")
+    return _native
+
+
+@pytest.fixture(params=golden_files(), ids=lambda p: os.path.basename(p)[7:-4])
+def golden(request):
+    return np.load(request.param, allow_pickle=False)
+
+
+def test_golden_tree_bit_exact(native, golden):
+    g = golden
+    for leaf in g["leaves"]:
+        dt = native.DeviceTree.build(g["pos"], g["mass"], int(leaf))
+        e = dt.export()
+        for f in TREE_FIELDS:
+            bits_equal(e[f], g[f"L{leaf}_{f}"], f"L{leaf} {f}")
+
+
+def test_golden_refit_bit_exact(native, golden):
+    g = golden
+    for leaf in g["leaves"]:
+        dt = native.DeviceTree.build(g["pos"], g["mass"], int(leaf))
+        dt.refit(g["pos2"], g["mass"])
+        e = dt.export()
+        bits_equal(e["mass"], g[f"L{leaf}_refit_mass"], "refit mass")
+        bits_equal(e["com"], g[f"L{leaf}_refit_com"], "refit com")
+        o = O.build(g["pos"], g["mass"], int(leaf))
+        bits_equal(e["center"], o.center, "center kept")
+        bits_equal(e["half_size"], o.half_size, "half_size kept")
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.01])
+def test_golden_lists_and_accelerations(native, golden, eps):
+    g = golden
+    pos, mass, ext = g["pos"], g["mass"], g["ext"]
+    for leaf in g["leaves"]:
+        dt = native.DeviceTree.build(pos, mass, int(leaf))
+        for theta in g["thetas"]:
+            tag = f"L{leaf}_t{theta}_e{eps}"
+            acc = dt.accel_self(theta, eps, 1.0)
+            nt = len(g[f"{tag}_acc_numba"])
+            acc_close(acc[:nt], g[f"{tag}_acc_numba"])
+            ptr, ids = dt.interaction_lists(None, theta)
+            gp, gi = g[f"{tag}_list_ptr"], g[f"{tag}_list_ids"]
+            for i in range(nt):
+                np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], gi[gp[i]:gp[i + 1]], err_msg=f"{tag} t{i}")
+            acc_e = dt.accel_targets(ext, theta, eps, 1.0)
+            acc_close(acc_e, g[f"{tag}_ext_acc"])
+            ptr, ids = dt.interaction_lists(ext, theta)
+            gp, gi = g[f"{tag}_ext_list_ptr"], g[f"{tag}_ext_list_ids"]
+            for i in range(len(ext)):
+                np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], gi[gp[i]:gp[i + 1]])
+
+
+def plummer(n, seed, a=1.0, rmax=100.0):
+    rng = np.random.default_rng(seed)
+    r = np.empty(n)
+    fill = 0
+    while fill < n:
+        u = rng.uniform(0, 1, n - fill)
+        rr = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+        rr = rr[rr < rmax]
+        r[fill:fill + len(rr)] = rr
+        fill += len(rr)
+    ct = rng.uniform(-1, 1, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    st = np.sqrt(1 - ct * ct)
+    return np.stack([r * st * np.cos(ph), r * st * np.sin(ph), r * ct], axis=1)
+
+
+@pytest.mark.parametrize("n,leaf,theta", [(10_000, 32, 0.5), (20_000, 7, 0.3), (65_537, 64, 0.7)])
+def test_medium_against_oracle(native, n, leaf, theta):
+    """Config-1 style inputs: tree bit-exact, lists and counters exact,
+    accelerations to 1e-12, against the oracle on the same seeded inputs."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, 3)) if n != 20_000 else plummer(n, 5)
+    m = np.ones(n) if n == 10_000 else rng.uniform(0.1, 2.0, n)
+    eps = 0.01
+    dt = native.DeviceTree.build(x, m, leaf)
+    ot = O.build(x, m, leaf)
+    e = dt.export()
+    for f in TREE_FIELDS:
+        bits_equal(e[f], getattr(ot, f), f)
+    acc = dt.accel_self(theta, eps, 1.0)
+    ref = O.accelerations(ot, x, m, softening=eps, theta=theta)
+    acc_close(acc, ref)
+    sel = rng.choice(n, 64, replace=False)
+    ptr, ids = dt.interaction_lists(None, theta)
+    for i in sel:
+        np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], O.target_list(ot, x[i], theta))
+    cnt = dt.counters(None, theta)
+    np.testing.assert_array_equal(cnt, O.counters(ot, x, self_mode=True, theta=theta))
+    tg = rng.uniform(-4, 4, (3000, 3))
+    acc_close(dt.accel_targets(tg, theta, eps, 1.0), O.accelerations(ot, x, m, tg, softening=eps, theta=theta))
+
+
+def test_two_dimensional(native):
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((5000, 2))
+    m = rng.uniform(0.5, 1.5, 5000)
+    dt = native.DeviceTree.build(x, m, 16)
+    ot = O.build(x, m, 16)
+    e = dt.export()
+    for f in TREE_FIELDS:
+        bits_equal(e[f], getattr(ot, f), f)
+    acc_close(dt.accel_self(0.5, 0.01, 1.0), O.accelerations(ot, x, m, softening=0.01, theta=0.5))
+
+
+def test_refit_matches_oracle_and_rebuild_rule(native):
+    import bhut_amd
+    rng = np.random.default_rng(8)
+    x = plummer(30_000, 8)
+    m = np.full(len(x), 1.0 / len(x))
+    t = bhut_amd.Tree(x, m).build()
+    x2 = x + 0.001 * rng.standard_normal(x.shape)
+    t.refit(x2)
+    ot = O.refit(O.build(x, m, 32), x2, m)
+    for f in ("mass", "com", "center", "half_size"):
+        bits_equal(getattr(t._tree, f), getattr(ot, f), f)
+    acc_close(t.accelerations(theta=0.5, softening=0.01), O.accelerations(ot, x2, m, softening=0.01, theta=0.5))
+    # a particle far outside 1.1 root half sizes forces a rebuild (refit.py:236-249)
+    x3 = x2.copy()
+    x3[0] = 1e4
+    assert not O.should_refit(ot, x3)
+    t.refit(x3)
+    ob = O.build(x3, m, 32)
+    bits_equal(t._tree.perm, ob.perm, "rebuilt perm")
+    bits_equal(t._tree.half_size, ob.half_size, "rebuilt half_size")
+
+
+def test_edge_cases(native):
+    import bhut_amd
+    # single particle: zero acceleration
+    a = bhut_amd.accelerations(np.array([[1.0, 2.0, 3.0]]), np.array([1.0]))
+    assert a.shape == (1, 3) and np.all(a == 0)
+    # coincident particles with eps = 0: Numba's ZeroDivisionError fallback
+    # to the pure-Python leaf gives zero for the coincident pair
+    x = np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 3.0], [2.0, 2.0, 3.0]])
+    m = np.array([1.0, 2.0, 3.0])
+    ot = O.build(x, m, 32)
+    acc_close(bhut_amd.accelerations(x, m), O.accelerations(ot, x, m, softening=0.0, theta=0.5))
+    # leaf_size larger than N, leaf_size 1, zero and negative masses
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((300, 3))
+    m = rng.uniform(-0.2, 1.0, 300)
+    m[:20] = 0.0
+    for leaf in (1, 3, 1000):
+        ot = O.build(x, m, leaf)
+        dt = native.DeviceTree.build(x, m, leaf)
+        e = dt.export()
+        for f in TREE_FIELDS:
+            bits_equal(e[f], getattr(ot, f), f"leaf {leaf} {f}")
+        acc_close(dt.accel_self(0.5, 0.05, 1.0), O.accelerations(ot, x, m, softening=0.05, theta=0.5))
+    # zero-mass system: every node skipped (bh.py:512-513)
+    dt = native.DeviceTree.build(x, np.zeros(300), 8)
+    assert np.all(dt.accel_self(0.5, 0.01, 1.0) == 0)
+
+
+def test_leaf_larger_than_pairwise_block(native):
+    """Leaves above 128 particles take NumPy's recursive pairwise split."""
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((3000, 3))
+    m = rng.uniform(0.0, 1.0, 3000) * 10.0 ** rng.integers(-6, 6, 3000)
+    for leaf in (200, 700):
+        dt = native.DeviceTree.build(x, m, leaf)
+        ot = O.build(x, m, leaf)
+        e = dt.export()
+        for f in ("mass", "com"):
+            bits_equal(e[f], getattr(ot, f), f"leaf {leaf} {f}")
+
+
+def test_full_size_plummer_properties(native):
+    """1M Plummer (the headline configuration): size-independent checks."""
+    n = 1_000_000
+    x = plummer(n, 1)
+    m = np.full(n, 1.0 / n)
+    dt = native.DeviceTree.build(x, m, 32)
+    e = dt.export()
+    assert e["mass"].shape == (65535,) and int(e["is_leaf"].sum()) == 32768
+    assert np.array_equal(np.sort(e["perm"]), np.arange(n))
+    np.testing.assert_array_equal(e["perm"], O.argsort_stable(O.morton_codes(x)))
+    ot = O.topology(n, 32)
+    for f in ("start", "count", "first_child"):
+        np.testing.assert_array_equal(e[f], getattr(ot, f))
+    np.testing.assert_allclose(e["mass"][0], 1.0, rtol=1e-12)
+    # sampled exact lists and 1e-12 accelerations against the oracle walk
+    full = O.build(x, m, 32)
+    for f in ("center", "half_size", "mass", "com"):
+        bits_equal(e[f], getattr(full, f), f)
+    rng = np.random.default_rng(1)
+    sel = rng.choice(n, 32, replace=False)
+    ptr, ids = dt.interaction_lists(None, 0.5)
+    for i in sel:
+        np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], O.target_list(full, x[i], 0.5))
+    acc = dt.accel_self(0.5, 0.01, 1.0)
+    sub = rng.choice(n, 2000, replace=False)
+    ref = O.accelerations(full, x, m, x[sub], softening=0.01, theta=0.5)
+    # external walk of these points = self walk apart from the self pair,
+    # which is exactly zero for eps > 0 only after dx = 0; compare with
+    # direct sums instead for the self targets
+    d = native.direct(x, m, x[sub], softening=0.01)
+    err = np.linalg.norm(acc[sub] - d, axis=1) / np.linalg.norm(d, axis=1)
+    assert np.sqrt(np.mean(err ** 2)) < 2e-2
+    acc_close(native.DeviceTree.accel_targets(dt, x[sub], 0.5, 0.01, 1.0), ref)
