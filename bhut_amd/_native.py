@@ -38,6 +38,8 @@ _SIGS = {
     "bh_ctx_stream": (vp, [vp]),
     "bh_ctx_set_stream": (ctypes.c_int, [vp, vp]),
     "bh_ctx_trim": (ctypes.c_int, [vp]),
+    "bh_ctx_profile": (ctypes.c_int, [vp, i32]),
+    "bh_ctx_profile_summary": (ctypes.c_int, [vp, ctypes.c_char_p, i64]),
     "bh_tree_build": (ctypes.c_int, [vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
     "bh_tree_build_dev": (ctypes.c_int, [vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
     "bh_tree_import": (ctypes.c_int, [vp, i64, i64, i32, i32] + [vp] * 13 + [ctypes.POINTER(vp)]),
@@ -118,6 +120,19 @@ class Context:
 
     def trim(self) -> None:
         check(load().bh_ctx_trim(self.handle))
+
+    def profile(self, enable: bool) -> None:
+        check(load().bh_ctx_profile(self.handle, int(bool(enable))))
+
+    def profile_summary(self) -> dict:
+        """{kernel name: (launches, total ms)} recorded since profile(True)."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(load().bh_ctx_profile_summary(self.handle, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
 
     def __del__(self):
         try:
@@ -232,6 +247,33 @@ class DeviceTree:
         check(load().bh_accel_self(self.handle, float(theta), float(softening), float(G), ptr(out)))
         return out
 
+    # -- device-pointer forms (inputs already resident in device memory) --
+    def accel_self_dev(self, out_ptr: int, theta: float, softening: float, G: float) -> None:
+        check(load().bh_accel_self_dev(self.handle, float(theta), float(softening), float(G), vp(out_ptr)))
+
+    def accel_targets_dev(self, tgt_ptr: int, m: int, out_ptr: int, theta: float, softening: float,
+                          G: float) -> None:
+        check(load().bh_accel_targets_dev(self.handle, vp(tgt_ptr), int(m), float(theta), float(softening),
+                                          float(G), vp(out_ptr)))
+
+    def accel_f32_self_dev(self, out_ptr: int, theta: float, softening: float, G: float,
+                           opts: F32Opts | None = None) -> np.ndarray:
+        st = np.zeros(4, np.int64)
+        check(load().bh_accel_f32_self_dev(self.handle, float(theta), float(softening), float(G),
+                                           ctypes.byref(opts) if opts is not None else None, vp(out_ptr), ptr(st)))
+        return st
+
+    def accel_f32_targets_dev(self, tgt_ptr: int, m: int, out_ptr: int, theta: float, softening: float, G: float,
+                              opts: F32Opts | None = None) -> np.ndarray:
+        st = np.zeros(4, np.int64)
+        check(load().bh_accel_f32_targets_dev(self.handle, vp(tgt_ptr), int(m), float(theta), float(softening),
+                                              float(G), ctypes.byref(opts) if opts is not None else None,
+                                              vp(out_ptr), ptr(st)))
+        return st
+
+    def refit_dev(self, pos_ptr: int, mass_ptr: int | None) -> None:
+        check(load().bh_tree_refit_dev(self.handle, vp(pos_ptr), vp(mass_ptr) if mass_ptr else None))
+
     def accel_targets(self, targets: np.ndarray, theta: float, softening: float, G: float) -> np.ndarray:
         tgt = f64c(targets)
         out = np.empty_like(tgt)
@@ -283,6 +325,14 @@ class DeviceTree:
                 self.handle = None
         except Exception:
             pass
+
+
+def direct_dev(src_ptr: int, mass_ptr: int, n: int, tgt_ptr: int, m: int, dim: int, self_mode: bool,
+               softening: float, G: float, acc_ptr: int | None, pot_ptr: int | None, context=None) -> None:
+    c = context or ctx()
+    check(load().bh_direct_dev(c.handle, vp(src_ptr), vp(mass_ptr), int(n), vp(tgt_ptr), int(m), int(dim),
+                               int(self_mode), float(softening), float(G), vp(acc_ptr) if acc_ptr else None,
+                               vp(pot_ptr) if pot_ptr else None))
 
 
 def direct(src, mass, targets=None, *, softening=0.0, G=1.0, potential=False, context=None):

@@ -77,6 +77,37 @@ void dfree(bh_ctx* c, void* p) {
   c->live.erase(it);
 }
 
+static hipEvent_t prof_event(bh_ctx* c) {
+  if (!c->prof_pool.empty()) {
+    hipEvent_t e = c->prof_pool.back();
+    c->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+ProfScope::ProfScope(bh_ctx* ctx, const char* name) : c(ctx) {
+  if (!c->prof) return;
+  bh_ctx::ProfRec r{name, prof_event(c), prof_event(c)};
+  (void)hipEventRecord(r.a, c->stream);
+  idx = c->prof_recs.size();
+  c->prof_recs.push_back(r);
+}
+
+ProfScope::~ProfScope() {
+  if (idx != (size_t)-1) (void)hipEventRecord(c->prof_recs[idx].b, c->stream);
+}
+
+static void prof_reset(bh_ctx* c) {
+  for (auto& r : c->prof_recs) {
+    c->prof_pool.push_back(r.a);
+    c->prof_pool.push_back(r.b);
+  }
+  c->prof_recs.clear();
+}
+
 struct DeviceGuard {
   int prev = -1;
   explicit DeviceGuard(int dev) {
@@ -146,20 +177,35 @@ static int build_dev(bh_ctx* c, const double* pos, const double* mass, int64_t n
   uint32_t* v1 = s.get<uint32_t>(n);
   if (!k0 || !k1 || !v1) return BH_ENOMEM;
   uint32_t* v0 = reinterpret_cast<uint32_t*>(t->perm);
-  BH_TRY(bbox(c, pos, n, dim, t->gbox));
-  BH_TRY(morton_iota(c, pos, n, dim, t->gbox, 0, k0, v0));
-  uint64_t* ko;
-  uint32_t* vo;
-  BH_TRY(radix_sort(c, k0, v0, n, 21 * dim, &ko, &vo, k1, v1));
-  if (vo != v0) BH_HIP(hipMemcpyAsync(v0, vo, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-  BH_TRY(gather(c, pos, mass, v0, n, dim, t->spart));
   // the (N, leaf) tables give M before the node arrays are allocated
   BH_TRY(topology_plan(n, leaf, &t->M, &t->nleaves, &t->depth));
   t->binary = true;
   BH_TRY(alloc_topology(t));
-  BH_TRY(topology(c, t));
-  BH_TRY(upward(c, t, 0));
-  BH_TRY(pack_nodes(c, t));
+  {
+    ProfScope p(c, "build_topology");
+    BH_TRY(topology(c, t));
+  }
+  {
+    ProfScope p(c, "build_bbox_morton");
+    BH_TRY(bbox(c, pos, n, dim, t->gbox));
+    BH_TRY(morton_iota(c, pos, n, dim, t->gbox, 0, k0, v0));
+  }
+  {
+    ProfScope p(c, "build_radix_sort");
+    uint64_t* ko;
+    uint32_t* vo;
+    BH_TRY(radix_sort(c, k0, v0, n, 21 * dim, &ko, &vo, k1, v1));
+    if (vo != v0) BH_HIP(hipMemcpyAsync(v0, vo, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+  }
+  {
+    ProfScope p(c, "build_gather");
+    BH_TRY(gather(c, pos, mass, v0, n, dim, t->spart));
+  }
+  {
+    ProfScope p(c, "build_upward");
+    BH_TRY(upward(c, t, 0));
+    BH_TRY(pack_nodes(c, t));
+  }
   return BH_OK;
 }
 
@@ -235,6 +281,38 @@ int bh_ctx_set_stream(bh_ctx* c, void* stream) {
   BH_HIP(hipStreamSynchronize(c->stream));
   c->stream = stream ? (hipStream_t)stream : c->own_stream;
   return BH_OK;
+}
+
+int bh_ctx_profile(bh_ctx* c, int32_t enable) {
+  BH_REQ(c, "ctx is NULL");
+  DeviceGuard g(c->device);
+  BH_HIP(hipStreamSynchronize(c->stream));
+  prof_reset(c);
+  c->prof = enable != 0;
+  return BH_OK;
+}
+
+int bh_ctx_profile_summary(bh_ctx* c, char* buf, int64_t cap) {
+  BH_REQ(c && buf && cap > 0, "NULL argument");
+  DeviceGuard g(c->device);
+  BH_HIP(hipStreamSynchronize(c->stream));
+  std::map<std::string, std::pair<int64_t, double>> agg;
+  for (auto& r : c->prof_recs) {
+    float ms = 0.f;
+    BH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    auto& e = agg[r.name];
+    e.first += 1;
+    e.second += ms;
+  }
+  std::string s;
+  char line[256];
+  for (auto& kv : agg) {
+    snprintf(line, sizeof(line), "%s %lld %.6f\n", kv.first.c_str(), (long long)kv.second.first,
+             kv.second.second);
+    s += line;
+  }
+  snprintf(buf, (size_t)cap, "%s", s.c_str());
+  return (int64_t)s.size() < cap ? BH_OK : BH_EOVERFLOW;
 }
 
 int bh_ctx_trim(bh_ctx* c) {

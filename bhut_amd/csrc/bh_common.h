@@ -75,7 +75,25 @@ struct bh_ctx {
   size_t pinned_bytes = 0;
   // FP32-form list capacities remembered between calls (entries per call)
   int64_t f32_mono_cap = 0, f32_leaf_cap = 0;
+  // optional per-kernel timing with HIP events on the context stream
+  bool prof = false;
+  struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_pool;
 };
+
+namespace bh {
+// Records HIP events around the launches in its scope when profiling is on.
+struct ProfScope {
+  bh_ctx* c;
+  size_t idx = (size_t)-1;
+  ProfScope(bh_ctx* ctx, const char* name);
+  ~ProfScope();
+};
+}  // namespace bh
 
 struct bh_tree {
   bh_ctx* ctx = nullptr;

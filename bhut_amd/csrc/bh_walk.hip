@@ -221,6 +221,8 @@ int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t 
     int rc = order_targets(c, t, tgt, m, torder);
     if (rc) return rc;
   }
+  static const char* names[] = {"k_walk_f64", "k_walk_f64_lcount", "k_walk_f64_lfill", "k_walk_f64_stats"};
+  ProfScope ps(c, names[mode & 3]);
   if (t->dim == 3) {
     if (tgt) launch_walk<3, false>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
     else launch_walk<3, true>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);

@@ -54,6 +54,12 @@ void *bh_ctx_stream(bh_ctx *ctx);             /* the hipStream_t used        */
  * stream); NULL returns to the context's own stream.  Synchronises first. */
 int bh_ctx_set_stream(bh_ctx *ctx, void *stream);
 int bh_ctx_trim(bh_ctx *ctx);                 /* release cached allocations  */
+/* Per-kernel timing with HIP events recorded on the context stream around
+ * the hot kernels (walks, list builds, evaluations).  enable=1 clears the
+ * records and starts recording; the summary is one "name launches total_ms"
+ * line per kernel (synchronises). */
+int bh_ctx_profile(bh_ctx *ctx, int32_t enable);
+int bh_ctx_profile_summary(bh_ctx *ctx, char *buf, int64_t cap);
 
 /* ------------------------------------------------------------------ build */
 /* build_tree (bhut/tree/build.py:19-107, via bhut/api.py:155-191): bounding

@@ -1,0 +1,115 @@
+"""The FP32 grouped form against FP64 direct summation and the plain form.
+
+Tolerances are stated per test: the FP32 form is not bit-faithful; at
+theta = 0 (every node opened, bh.py:600-601) it is an FP32 direct sum whose
+relative error is a few 1e-7, and at theta > 0 its conservative group test
+makes it at least as accurate as the plain FP64 walk (rms relative error
+against FP64 direct summation).
+"""
+
+import numpy as np
+import pytest
+
+from bhut_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def native():
+    from bhut_amd import _native
+    if _native.device_count() < 1:
+        pytest.skip("no HIP device")
+    return _native
+
+
+def rel_err(a, ref):
+    e = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return float(np.sqrt(np.mean(e ** 2))), float(e.max())
+
+
+def test_theta_zero_is_fp32_direct_sum(native):
+    x, m = workloads.plummer(20_000, seed=3)
+    dt = native.DeviceTree.build(x, m, 32)
+    a = dt.accel_f32(None, 0.0, 0.01, 1.0)
+    d = native.direct(x, m, softening=0.01)
+    rms, mx = rel_err(a, d)
+    assert rms < 2e-6 and mx < 2e-5, (rms, mx)
+
+
+@pytest.mark.parametrize("gs,kahan", [(64, 1), (128, 1), (64, 0)])
+def test_theta_half_accuracy(native, gs, kahan):
+    n = 100_000
+    x, m = workloads.plummer(n, seed=1)
+    dt = native.DeviceTree.build(x, m, 32)
+    opts = native.F32Opts(gs, kahan, 0, 0)
+    a32, st = dt.accel_f32(None, 0.5, 0.01, 1.0, opts, stats=True)
+    a64 = dt.accel_self(0.5, 0.01, 1.0)
+    sel = np.random.default_rng(0).choice(n, 2000, replace=False)
+    d = native.direct(x, m, x[sel], softening=0.01)
+    rms32, _ = rel_err(a32[sel], d)
+    rms64, _ = rel_err(a64[sel], d)
+    # the group test only ever opens more than the per-target test
+    assert rms32 <= rms64 * 1.05, (rms32, rms64)
+    assert rms32 < 3e-3
+    assert st[0] > 0 and st[1] > 0 and st[2] == -(-n // gs)
+
+
+def test_external_targets(native):
+    x, m = workloads.plummer(50_000, seed=2)
+    dt = native.DeviceTree.build(x, m, 16)
+    tg = np.random.default_rng(2).uniform(-5, 5, (20_000, 3))
+    a0 = dt.accel_f32(tg, 0.0, 0.01, 1.0)
+    d = native.direct(x, m, tg, softening=0.01)
+    rms, mx = rel_err(a0, d)
+    assert rms < 2e-6 and mx < 2e-5
+    a = dt.accel_f32(tg, 0.5, 0.01, 1.0)
+    a64 = dt.accel_targets(tg, 0.5, 0.01, 1.0)
+    assert rel_err(a, d)[0] <= rel_err(a64, d)[0] * 1.05
+
+
+def test_two_dimensional_and_zero_softening(native):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((8000, 2))
+    m = rng.uniform(0.5, 1.5, 8000)
+    dt = native.DeviceTree.build(x, m, 16)
+    for eps in (0.01, 0.0):
+        a = dt.accel_f32(None, 0.0, eps, 1.0)
+        d = native.direct(x, m, softening=eps)
+        rms, _ = rel_err(a, d)
+        assert rms < 1e-5, (eps, rms)
+
+
+def test_refit_then_fp32(native):
+    x, m = workloads.plummer(30_000, seed=6)
+    dt = native.DeviceTree.build(x, m, 32)
+    dt.accel_f32(None, 0.5, 0.01, 1.0)
+    x2 = x + 1e-3 * np.random.default_rng(6).standard_normal(x.shape)
+    dt.refit(x2, m)
+    a = dt.accel_f32(None, 0.0, 0.01, 1.0)
+    rms, _ = rel_err(a, native.direct(x2, m, softening=0.01))
+    assert rms < 2e-6
+
+
+def test_stack_overflow_is_reported(native):
+    """A tiny LDS stack bound, forced from the host: the walk kernel's own
+    bound check stops the group and the call raises, no fault."""
+    x, m = workloads.plummer(50_000, seed=7)
+    dt = native.DeviceTree.build(x, m, 8)
+    with pytest.raises(RuntimeError, match="stack"):
+        dt.accel_f32(None, 0.3, 0.01, 1.0, native.F32Opts(64, 1, 64, 0))
+    # the tree is still usable afterwards
+    a = dt.accel_f32(None, 0.3, 0.01, 1.0)
+    assert np.all(np.isfinite(a))
+
+
+def test_api_precision_keyword(native):
+    import bhut_amd
+    x, m = workloads.plummer(5000, seed=9)
+    a32 = bhut_amd.accelerations(x, m, theta=0.0, softening=0.01, precision="fp32")
+    a64 = bhut_amd.accelerations(x, m, theta=0.0, softening=0.01)
+    assert rel_err(a32, a64)[0] < 2e-6
+    t = bhut_amd.Tree(x, m).build()
+    assert rel_err(t.accelerations(theta=0.0, softening=0.01, precision="fp32"), a64)[0] < 2e-6
+    with pytest.raises(ValueError, match="precision"):
+        t.accelerations(precision="fp16")
