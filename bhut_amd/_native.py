@@ -24,7 +24,7 @@ i64, i32, dbl, vp = ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_vo
 
 
 class F32Opts(ctypes.Structure):
-    _fields_ = [("group_size", i32), ("kahan", i32), ("stack_cap", i32), ("reserved", i32)]
+    _fields_ = [("group_size", i32), ("kahan", i32), ("stack_cap", i32), ("multipole", i32)]
 
 
 # name -> (restype, argtypes); mirrors include/bhut_hip.h

@@ -138,7 +138,7 @@ static void free_tree_arrays(bh_tree* t) {
   bh_ctx* c = t->ctx;
   void* ps[] = {t->spart, t->perm,  t->start, t->count, t->first_child, t->parent, t->skip,
                 t->leaf_order, t->center, t->half, t->mass, t->com, t->bmin, t->bmax, t->gbox,
-                t->nodes, t->spart32};
+                t->nodes, t->spart32, t->pleaf, t->quad};
   for (void* p : ps) dfree(c, p);
 }
 

@@ -124,9 +124,13 @@ struct bh_tree {
   double* bmax = nullptr;
   double* gbox = nullptr;     // 2*dim global pmin/pmax
   bh::NodeD* nodes = nullptr; // packed traversal records
-  // FP32 form: particles relative to their leaf centre
+  // FP32 form: particles relative to their leaf centre, each particle's
+  // leaf, and the nodes' second moments about their centre of mass
   float4* spart32 = nullptr;
+  int32_t* pleaf = nullptr;
+  double* quad = nullptr;  // M*6: xx, yy, zz, xy, xz, yz
   bool f32_ready = false;
+  bool quad_ready = false;
 };
 
 namespace bh {

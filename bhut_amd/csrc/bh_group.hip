@@ -1,29 +1,31 @@
 // bh_group.hip -- the FP32 grouped form (the chosen, fast form).
 //
-// Same tree, same monopole-plus-leaf physics as bhut (bhut/traverse/bh.py:
-// 485-548: leaves are always summed particle by particle, internal nodes are
-// accepted as monopoles or opened), evaluated for groups of GS Morton-
-// contiguous targets at once:
+// Same tree and the same leaf rule as bhut (bhut/traverse/bh.py:485-548:
+// leaves are always summed particle by particle, internal nodes are accepted
+// or opened), evaluated for groups of GS Morton-contiguous targets at once:
 //
 //  1. k_group_box: FP64 bounding box of each group's targets.
-//  2. k_group_walk: one wavefront per group walks the tree breadth-wise
-//     with an LDS stack: 64 nodes are tested per step, one per lane, with a
+//  2. k_group_walk: one wavefront per group walks the tree breadth-wise with
+//     an LDS stack: 64 nodes are tested per step, one per lane, with a
 //     conservative group test -- accept a node only if size < theta * dmin,
 //     dmin the distance from its centre of mass to the group box, so every
-//     target of the group would accept it under bhut's own test
-//     (bh.py:586-601).  Lanes are compacted with ballots into the monopole
-//     list, the leaf list and the stack.  Run twice: count, then fill.
-//  3. k_group_eval: one wavefront per group, one (or two) targets per lane,
-//     over the lists in HBM.  List entries and source particles sit at
-//     wave-uniform addresses and are fetched with scalar loads; every
-//     interaction is FP32 on coordinates relative to the group centre, with
-//     sources stored relative to their leaf centre and the leaf-to-group
-//     shift formed in FP64, so precision follows the size of the leaf, not
-//     the size of the coordinates.  Per-leaf (and per-32-monopole) partial
-//     sums are added to the per-target total with Kahan compensation.
+//     target of the group would also accept it under bhut's own test
+//     (bh.py:586-601).  Lanes are compacted with ballots into the cell list,
+//     the leaf-chunk list and the stack.  Run twice: count, then fill.
+//  3. k_group_eval: one wavefront per group, one (or two) targets per lane.
+//     Tiles of 64 list entries are loaded cooperatively (coalesced) into LDS
+//     and read back as wave-wide broadcasts, double-buffered so the next
+//     tile's HBM reads overlap this tile's arithmetic.  All interactions are
+//     FP32 on coordinates relative to the group centre: sources are stored
+//     relative to their leaf centre and shifted by the FP64-formed
+//     leaf-to-group offset, so precision follows the leaf size, not the
+//     coordinate size.  Each tile's partial sum is added to the per-target
+//     total with Kahan compensation.
 //
-// FP64 stays in the tree build (box, quantisation, mass and centre of mass)
-// and in the group-centre/leaf-shift subtractions.
+// Accepted cells carry their mass, centre of mass and second moments
+// (quadrupole, accumulated in FP64 by k_quad_up); the monopole-only variant
+// is kept as an option.  FP64 stays in the tree build, the moments, the group
+// box and the centre/shift subtractions.
 #include "bh_common.h"
 
 namespace bh {
@@ -32,23 +34,31 @@ struct GroupBox {
   double cx, cy, cz, hx, hy, hz;
 };
 
-struct __attribute__((aligned(16))) LeafEnt {
+// Accepted cell: com - group centre, mass; second moments about the com.
+struct __attribute__((aligned(16))) CellEnt {
+  float4 a;  // x, y, z, m
+  float4 b;  // qxx, qyy, qzz, trace
+  float4 c;  // qxy, qxz, qyz, 0
+};
+
+// A run of at most CH particles of one leaf.
+struct __attribute__((aligned(16))) ChunkEnt {
   float sx, sy, sz;  // leaf centre - group centre
-  int32_t start;
-  int32_t count;
+  int32_t start;     // first particle (sorted order)
+  int32_t count;     // particles in this chunk
   int32_t pad[3];
 };
-static_assert(sizeof(LeafEnt) == 32, "LeafEnt layout");
+static_assert(sizeof(ChunkEnt) == 32, "ChunkEnt layout");
 
 // ---------------------------------------------------------------- prepare
-// Source particles relative to their leaf's centre, FP32.
+// Sources relative to their leaf's centre (FP32) and each particle's leaf.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_src32(int64_t nleaves, const int32_t* __restrict__ leaf_order,
                                                const int32_t* __restrict__ start,
                                                const int32_t* __restrict__ count,
                                                const double* __restrict__ center,
                                                const double4* __restrict__ spart,
-                                               float4* __restrict__ src32) {
+                                               float4* __restrict__ src32, int32_t* __restrict__ pleaf) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nleaves) return;
   const int v = leaf_order[t];
@@ -57,22 +67,91 @@ __global__ __launch_bounds__(256) void k_src32(int64_t nleaves, const int32_t* _
   for (int64_t j = s; j < s + c; j++) {
     const double4 p = spart[j];
     src32[j] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz), (float)p.w);
+    pleaf[j] = v;
+  }
+}
+
+// Second moments Q = sum m (x - com)(x - com)^T, FP64, by the same
+// atomic-flag climb as the build's upward pass: leaves from their particles,
+// internal nodes by the parallel-axis shift of their children's moments.
+__device__ __forceinline__ bool arrive_q(uint32_t* flags, int node) {
+  return __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+__global__ __launch_bounds__(256) void k_quad_up(int64_t nleaves, const int32_t* __restrict__ leaf_order,
+                                                 const int32_t* __restrict__ start,
+                                                 const int32_t* __restrict__ count,
+                                                 const int32_t* __restrict__ first_child,
+                                                 const int32_t* __restrict__ parent, const double4* spart,
+                                                 const double* com, int dim, const double* mass, double* quad,
+                                                 uint32_t* flags) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nleaves) return;
+  const int v = leaf_order[t];
+  {
+    const double cx = com[v * dim], cy = com[v * dim + 1], cz = dim == 3 ? com[v * dim + 2] : 0.0;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    const int64_t s = start[v], c = count[v];
+    for (int64_t j = s; j < s + c; j++) {
+      const double4 p = spart[j];
+      const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+      q[0] += p.w * dx * dx;
+      q[1] += p.w * dy * dy;
+      q[2] += p.w * dz * dz;
+      q[3] += p.w * dx * dy;
+      q[4] += p.w * dx * dz;
+      q[5] += p.w * dy * dz;
+    }
+    for (int k = 0; k < 6; k++) quad[(int64_t)v * 6 + k] = q[k];
+  }
+  int node = parent[v];
+  while (node >= 0) {
+    if (!arrive_q(flags, node)) return;
+    const double cx = com[node * dim], cy = com[node * dim + 1], cz = dim == 3 ? com[node * dim + 2] : 0.0;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    const int a = first_child[node];
+    for (int ch = a; ch < a + 2; ch++) {
+      const double m = mass[ch];
+      const double dx = com[ch * dim] - cx, dy = com[ch * dim + 1] - cy, dz = dim == 3 ? com[ch * dim + 2] - cz : 0.0;
+      const double* qc = quad + (int64_t)ch * 6;
+      q[0] += qc[0] + m * dx * dx;
+      q[1] += qc[1] + m * dy * dy;
+      q[2] += qc[2] + m * dz * dz;
+      q[3] += qc[3] + m * dx * dy;
+      q[4] += qc[4] + m * dx * dz;
+      q[5] += qc[5] + m * dy * dz;
+    }
+    for (int k = 0; k < 6; k++) quad[(int64_t)node * 6 + k] = q[k];
+    node = parent[node];
   }
 }
 
 int f32_prepare(bh_ctx* c, bh_tree* t) {
-  if (!t->spart32) {
-    t->spart32 = dalloc_t<float4>(c, (size_t)t->n);
-    if (!t->spart32) return BH_ENOMEM;
-  }
+  if (!t->spart32) t->spart32 = dalloc_t<float4>(c, (size_t)t->n);
+  if (!t->pleaf) t->pleaf = dalloc_t<int32_t>(c, (size_t)t->n);
+  if (!t->quad) t->quad = dalloc_t<double>(c, (size_t)t->M * 6);
+  if (!t->spart32 || !t->pleaf || !t->quad) return BH_ENOMEM;
   const unsigned nb = (unsigned)cdiv(t->nleaves, 256);
   if (t->dim == 3)
     hipLaunchKernelGGL(k_src32<3>, dim3(nb), dim3(256), 0, c->stream, t->nleaves, t->leaf_order, t->start,
-                       t->count, t->center, t->spart, t->spart32);
+                       t->count, t->center, t->spart, t->spart32, t->pleaf);
   else
     hipLaunchKernelGGL(k_src32<2>, dim3(nb), dim3(256), 0, c->stream, t->nleaves, t->leaf_order, t->start,
-                       t->count, t->center, t->spart, t->spart32);
+                       t->count, t->center, t->spart, t->spart32, t->pleaf);
   BH_LAUNCH_CHECK();
+  if (t->binary) {
+    Scratch s(c);
+    uint32_t* flags = s.get<uint32_t>((size_t)t->M);
+    if (!flags) return BH_ENOMEM;
+    int rc = fill_u32(c, flags, 0u, t->M);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_quad_up, dim3(nb), dim3(256), 0, c->stream, t->nleaves, t->leaf_order, t->start,
+                       t->count, t->first_child, t->parent, t->spart, t->com, t->dim, t->mass, t->quad, flags);
+    BH_LAUNCH_CHECK();
+    t->quad_ready = true;
+  } else {
+    t->quad_ready = false;  // imported trees with other fan-outs: monopoles only
+  }
   t->f32_ready = true;
   return BH_OK;
 }
@@ -126,15 +205,14 @@ __global__ void k_gather_targets(const double* __restrict__ tgt, const uint32_t*
 // ---------------------------------------------------------------- walk
 template <int DIM, bool FILL>
 __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nodes,
-                                                   const double* __restrict__ center, int64_t ngroups,
+                                                   const double* __restrict__ center,
+                                                   const double* __restrict__ quad, int use_quad,
                                                    const GroupBox* __restrict__ boxes, double theta2,
-                                                   int stack_cap, int64_t* __restrict__ mcount,
-                                                   int64_t* __restrict__ lcount,
-                                                   int64_t* __restrict__ pcount,
-                                                   const int64_t* __restrict__ moff,
-                                                   const int64_t* __restrict__ loff,
-                                                   float4* __restrict__ mlist,
-                                                   LeafEnt* __restrict__ llist,
+                                                   int stack_cap, int ch, int64_t* __restrict__ ccount,
+                                                   int64_t* __restrict__ kcount, int64_t* __restrict__ pcount,
+                                                   const int64_t* __restrict__ coff,
+                                                   const int64_t* __restrict__ koff,
+                                                   CellEnt* __restrict__ clist, ChunkEnt* __restrict__ klist,
                                                    uint32_t* __restrict__ err) {
   extern __shared__ int32_t stack[];
   const int64_t g = blockIdx.x;
@@ -143,9 +221,9 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
   if (lane == 0) stack[0] = 0;
   __syncthreads();
   int sp = 1;
-  int64_t nm = 0, nl = 0, np = 0;
-  const int64_t mb = FILL ? moff[g] : 0, me = FILL ? moff[g + 1] : 0;
-  const int64_t lb = FILL ? loff[g] : 0, le = FILL ? loff[g + 1] : 0;
+  int64_t nc = 0, nk = 0, np = 0;
+  const int64_t cb = FILL ? coff[g] : 0, ce = FILL ? coff[g + 1] : 0;
+  const int64_t kb = FILL ? koff[g] : 0, ke = FILL ? koff[g + 1] : 0;
   const uint64_t lt = lanemask_lt();
   while (sp > 0) {
     const int k = sp < 64 ? sp : 64;
@@ -153,13 +231,15 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
     if (lane < k) v = stack[sp - k + lane];
     __syncthreads();
     sp -= k;
-    int kind = 0;  // 1 monopole, 2 leaf, 3 open
+    int kind = 0;  // 1 cell, 2 leaf, 3 open
+    int nchunk = 0;
     NodeD nd;
     if (v >= 0) {
       nd = nodes[v];
       if (nd.m != 0.0) {
         if (nd.fc < 0) {
           kind = 2;
+          nchunk = (nd.count + ch - 1) / ch;
         } else {
           double ex = fabs(nd.cx - B.cx) - B.hx;
           double ey = fabs(nd.cy - B.cy) - B.hy;
@@ -173,7 +253,16 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
         }
       }
     }
-    const uint64_t bm = __ballot(kind == 1), bl = __ballot(kind == 2), bo = __ballot(kind == 3);
+    const uint64_t bc = __ballot(kind == 1), bo = __ballot(kind == 3);
+    // exclusive prefix of chunk counts over the lanes (wave scan)
+    int kx = nchunk;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(kx, off);
+      if (lane >= off) kx += y;
+    }
+    const int ktot = __shfl(kx, 63);
+    kx -= nchunk;
     if (kind == 3) {
       const int o = sp + 2 * __popcll(bo & lt);
       if (o + 1 < stack_cap) {
@@ -183,32 +272,47 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
     }
     if (FILL) {
       if (kind == 1) {
-        const int64_t o = mb + nm + __popcll(bm & lt);
-        if (o < me)
-          mlist[o] = make_float4((float)(nd.cx - B.cx), (float)(nd.cy - B.cy), (float)(nd.cz - B.cz), (float)nd.m);
+        const int64_t o = cb + nc + __popcll(bc & lt);
+        if (o < ce) {
+          CellEnt e;
+          e.a = make_float4((float)(nd.cx - B.cx), (float)(nd.cy - B.cy), (float)(nd.cz - B.cz), (float)nd.m);
+          if (use_quad) {
+            const double* q = quad + (int64_t)v * 6;
+            e.b = make_float4((float)q[0], (float)q[1], (float)q[2], (float)(q[0] + q[1] + q[2]));
+            e.c = make_float4((float)q[3], (float)q[4], (float)q[5], 0.f);
+          } else {
+            e.b = make_float4(0.f, 0.f, 0.f, 0.f);
+            e.c = make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+          clist[o] = e;
+        }
       }
       if (kind == 2) {
-        const int64_t o = lb + nl + __popcll(bl & lt);
-        if (o < le) {
-          LeafEnt e;
-          e.sx = (float)(center[(int64_t)v * DIM] - B.cx);
-          e.sy = (float)(center[(int64_t)v * DIM + 1] - B.cy);
-          e.sz = DIM == 3 ? (float)(center[(int64_t)v * DIM + 2] - B.cz) : 0.f;
-          e.start = nd.start;
-          e.count = nd.count;
-          e.pad[0] = e.pad[1] = e.pad[2] = 0;
-          llist[o] = e;
+        const float sx = (float)(center[(int64_t)v * DIM] - B.cx);
+        const float sy = (float)(center[(int64_t)v * DIM + 1] - B.cy);
+        const float sz = DIM == 3 ? (float)(center[(int64_t)v * DIM + 2] - B.cz) : 0.f;
+        for (int q = 0; q < nchunk; q++) {
+          const int64_t o = kb + nk + kx + q;
+          if (o < ke) {
+            ChunkEnt e;
+            e.sx = sx;
+            e.sy = sy;
+            e.sz = sz;
+            e.start = nd.start + q * ch;
+            e.count = min(ch, nd.count - q * ch);
+            e.pad[0] = e.pad[1] = e.pad[2] = 0;
+            klist[o] = e;
+          }
         }
       }
     } else {
-      // particles summed directly: wave total of leaf counts
       int pc = kind == 2 ? nd.count : 0;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
       np += pc;
     }
-    nm += __popcll(bm);
-    nl += __popcll(bl);
+    nc += __popcll(bc);
+    nk += ktot;
     const int push = 2 * __popcll(bo);
     if (sp + push > stack_cap) {
       if (lane == 0) atomicOr(err, 1u);
@@ -218,23 +322,56 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
     __syncthreads();
   }
   if (!FILL && lane == 0) {
-    mcount[g] = nm;
-    lcount[g] = nl;
+    ccount[g] = nc;
+    kcount[g] = nk;
     pcount[g] = np;
   }
 }
 
 // ---------------------------------------------------------------- eval
 template <bool ZEPS>
-__device__ __forceinline__ void interact(float dx, float dy, float dz, float m, float eps2, float& px, float& py,
-                                         float& pz) {
+__device__ __forceinline__ void p2p(float4 s, float tx, float ty, float tz, float eps2, float& px, float& py,
+                                    float& pz) {
+  const float dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
   float ri = __builtin_amdgcn_rsqf(r2);
   if (ZEPS) ri = r2 > 0.f ? ri : 0.f;
-  const float mg = m * (ri * ri * ri);
+  const float mg = s.w * (ri * ri * ri);
   px = fmaf(mg, dx, px);
   py = fmaf(mg, dy, py);
   pz = fmaf(mg, dz, pz);
+}
+
+// a = u (M r^-3 - 1.5 tr r^-5 + 7.5 uQu r^-7) - 3 r^-5 Q u, u = com - target:
+// the monopole plus quadrupole field of the cell's second moments about its
+// centre of mass (softened radius throughout).
+template <bool ZEPS, bool QUAD>
+__device__ __forceinline__ void cell(const float4& a, const float4& b, const float4& c, float tx, float ty, float tz,
+                                     float eps2, float& px, float& py, float& pz) {
+  const float ux = a.x - tx, uy = a.y - ty, uz = a.z - tz;
+  const float r2 = fmaf(ux, ux, fmaf(uy, uy, fmaf(uz, uz, eps2)));
+  float ri = __builtin_amdgcn_rsqf(r2);
+  if (ZEPS) ri = r2 > 0.f ? ri : 0.f;
+  const float ri2 = ri * ri;
+  const float ri3 = ri2 * ri;
+  if (!QUAD) {
+    const float mg = a.w * ri3;
+    px = fmaf(mg, ux, px);
+    py = fmaf(mg, uy, py);
+    pz = fmaf(mg, uz, pz);
+    return;
+  }
+  const float ri5 = ri3 * ri2;
+  const float ri7 = ri5 * ri2;
+  const float qx = fmaf(b.x, ux, fmaf(c.x, uy, c.y * uz));
+  const float qy = fmaf(c.x, ux, fmaf(b.y, uy, c.z * uz));
+  const float qz = fmaf(c.y, ux, fmaf(c.z, uy, b.z * uz));
+  const float uqu = fmaf(ux, qx, fmaf(uy, qy, uz * qz));
+  const float cu = fmaf(7.5f * uqu, ri7, fmaf(-1.5f * b.w, ri5, a.w * ri3));
+  const float cq = -3.f * ri5;
+  px = fmaf(cu, ux, fmaf(cq, qx, px));
+  py = fmaf(cu, uy, fmaf(cq, qy, py));
+  pz = fmaf(cu, uz, fmaf(cq, qz, pz));
 }
 
 __device__ __forceinline__ void kahan_add(float v, float& s, float& c) {
@@ -244,16 +381,39 @@ __device__ __forceinline__ void kahan_add(float v, float& s, float& c) {
   s = t;
 }
 
-template <int TPL, bool ZEPS, bool SELF, bool KAHAN>
+template <int TPL, bool KAHAN>
+__device__ __forceinline__ void flush(float (&px)[TPL], float (&py)[TPL], float (&pz)[TPL], float (&sx)[TPL],
+                                      float (&sy)[TPL], float (&sz)[TPL], float (&cx)[TPL], float (&cy)[TPL],
+                                      float (&cz)[TPL]) {
+#pragma unroll
+  for (int q = 0; q < TPL; q++) {
+    if (KAHAN) {
+      kahan_add(px[q], sx[q], cx[q]);
+      kahan_add(py[q], sy[q], cy[q]);
+      kahan_add(pz[q], sz[q], cz[q]);
+    } else {
+      sx[q] += px[q];
+      sy[q] += py[q];
+      sz[q] += pz[q];
+    }
+    px[q] = py[q] = pz[q] = 0.f;
+  }
+}
+
+// CH: particles per chunk (64/CH chunks fill one 64-slot tile).
+template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
 __global__ __launch_bounds__(64) void k_group_eval(int64_t m, int gs, const double4* __restrict__ tpos,
                                                    const uint32_t* __restrict__ torig,
+                                                   const float4* __restrict__ src32,
+                                                   const int32_t* __restrict__ pleaf,
+                                                   const double* __restrict__ center, int dim,
                                                    const GroupBox* __restrict__ boxes,
-                                                   const int64_t* __restrict__ moff,
-                                                   const float4* __restrict__ mlist,
-                                                   const int64_t* __restrict__ loff,
-                                                   const LeafEnt* __restrict__ llist,
-                                                   const float4* __restrict__ src32, float eps2, double G,
-                                                   int dim, double* __restrict__ out) {
+                                                   const int64_t* __restrict__ coff,
+                                                   const CellEnt* __restrict__ clist,
+                                                   const int64_t* __restrict__ koff,
+                                                   const ChunkEnt* __restrict__ klist, float eps2, double G,
+                                                   double* __restrict__ out) {
+  __shared__ float4 tile[2][3][64];
   const int64_t g = blockIdx.x;
   const int lane = threadIdx.x;
   const int64_t t0 = g * gs;
@@ -261,92 +421,85 @@ __global__ __launch_bounds__(64) void k_group_eval(int64_t m, int gs, const doub
   const GroupBox B = boxes[g];
   float tx[TPL], ty[TPL], tz[TPL];
   int64_t ti[TPL];
-  float sx[TPL], sy[TPL], sz[TPL], cx[TPL], cy[TPL], cz[TPL];
+  float sx[TPL], sy[TPL], sz[TPL], cx[TPL], cy[TPL], cz[TPL], px[TPL], py[TPL], pz[TPL];
 #pragma unroll
   for (int q = 0; q < TPL; q++) {
     ti[q] = t0 + lane + 64 * q;
-    const double4 p = tpos[ti[q] < t1 ? ti[q] : t0];
-    tx[q] = (float)(p.x - B.cx);
-    ty[q] = (float)(p.y - B.cy);
-    tz[q] = (float)(p.z - B.cz);
-    sx[q] = sy[q] = sz[q] = 0.f;
-    cx[q] = cy[q] = cz[q] = 0.f;
-  }
-  // monopoles, flushed every 32
-  const int64_t m0 = moff[g], m1 = moff[g + 1];
-  for (int64_t e = m0; e < m1; e += 32) {
-    const int64_t ee = e + 32 < m1 ? e + 32 : m1;
-    float px[TPL], py[TPL], pz[TPL];
-#pragma unroll
-    for (int q = 0; q < TPL; q++) px[q] = py[q] = pz[q] = 0.f;
-#pragma unroll 4
-    for (int64_t k = e; k < ee; k++) {
-      const float4 c = mlist[k];
-#pragma unroll
-      for (int q = 0; q < TPL; q++) interact<ZEPS>(c.x - tx[q], c.y - ty[q], c.z - tz[q], c.w, eps2, px[q], py[q], pz[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < TPL; q++) {
-      if (KAHAN) {
-        kahan_add(px[q], sx[q], cx[q]);
-        kahan_add(py[q], sy[q], cy[q]);
-        kahan_add(pz[q], sz[q], cz[q]);
-      } else {
-        sx[q] += px[q];
-        sy[q] += py[q];
-        sz[q] += pz[q];
-      }
-    }
-  }
-  // leaves
-  const int64_t l0 = loff[g], l1 = loff[g + 1];
-  for (int64_t e = l0; e < l1; e++) {
-    const LeafEnt L = llist[e];
-    float ux[TPL], uy[TPL], uz[TPL], px[TPL], py[TPL], pz[TPL];
-#pragma unroll
-    for (int q = 0; q < TPL; q++) {
-      ux[q] = tx[q] - L.sx;
-      uy[q] = ty[q] - L.sy;
-      uz[q] = tz[q] - L.sz;
-      px[q] = py[q] = pz[q] = 0.f;
-    }
-    const int32_t j0 = L.start, j1 = L.start + L.count;
-    const bool overlap = SELF && j0 < t1 && t0 < j1;
-    if (!overlap) {
-#pragma unroll 4
-      for (int32_t j = j0; j < j1; j++) {
-        const float4 s = src32[j];
-#pragma unroll
-        for (int q = 0; q < TPL; q++) interact<ZEPS>(s.x - ux[q], s.y - uy[q], s.z - uz[q], s.w, eps2, px[q], py[q], pz[q]);
-      }
+    const int64_t i = ti[q] < t1 ? ti[q] : t0;
+    if (SELF) {
+      // built exactly like the source copy of the same particle, so its own
+      // pair has dx == 0 and contributes nothing
+      const float4 s = src32[i];
+      const int v = pleaf[i];
+      tx[q] = s.x + (float)(center[(int64_t)v * dim] - B.cx);
+      ty[q] = s.y + (float)(center[(int64_t)v * dim + 1] - B.cy);
+      tz[q] = s.z + (dim == 3 ? (float)(center[(int64_t)v * dim + 2] - B.cz) : 0.f);
     } else {
-      for (int32_t j = j0; j < j1; j++) {
-        const float4 s = src32[j];
+      const double4 p = tpos[i];
+      tx[q] = (float)(p.x - B.cx);
+      ty[q] = (float)(p.y - B.cy);
+      tz[q] = (float)(p.z - B.cz);
+    }
+    sx[q] = sy[q] = sz[q] = cx[q] = cy[q] = cz[q] = px[q] = py[q] = pz[q] = 0.f;
+  }
+
+  // ---- accepted cells: tiles of 64 entries
+  const int64_t c0 = coff[g], c1 = coff[g + 1];
+  if (c1 > c0) {
+    CellEnt nxt;
+    int64_t e = c0;
+    if (e + lane < c1) nxt = clist[e + lane];
+    int buf = 0;
+    for (; e < c1; e += 64) {
+      tile[buf][0][lane] = nxt.a;
+      tile[buf][1][lane] = nxt.b;
+      tile[buf][2][lane] = nxt.c;
+      __syncthreads();
+      if (e + 64 + lane < c1) nxt = clist[e + 64 + lane];
+      const int cnt = (int)(c1 - e < 64 ? c1 - e : 64);
+      for (int k = 0; k < cnt; k++) {
+        const float4 a = tile[buf][0][k], b = tile[buf][1][k], c = tile[buf][2][k];
 #pragma unroll
-        for (int q = 0; q < TPL; q++) {
-          const float w = j == ti[q] ? 0.f : s.w;  // skip the target itself
-          const float dx = s.x - ux[q], dy = s.y - uy[q], dz = s.z - uz[q];
-          const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-          float ri = __builtin_amdgcn_rsqf(r2);
-          ri = (j == ti[q] || (ZEPS && !(r2 > 0.f))) ? 0.f : ri;
-          const float mg = w * (ri * ri * ri);
-          px[q] = fmaf(mg, dx, px[q]);
-          py[q] = fmaf(mg, dy, py[q]);
-          pz[q] = fmaf(mg, dz, pz[q]);
+        for (int q = 0; q < TPL; q++) cell<ZEPS, QUAD>(a, b, c, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
+      }
+      flush<TPL, KAHAN>(px, py, pz, sx, sy, sz, cx, cy, cz);
+      buf ^= 1;
+    }
+  }
+
+  // ---- leaf chunks: tiles of 64/CH chunks, lane l holds particle l%CH of
+  // chunk l/CH (padding slots carry zero mass far away)
+  constexpr int CPT = 64 / CH;
+  const int64_t k0 = koff[g], k1 = koff[g + 1];
+  if (k1 > k0) {
+    const int sub = lane / CH, off = lane % CH;
+    auto load = [&](int64_t e) -> float4 {
+      const int64_t ce = e + sub;
+      float4 r = make_float4(1e15f, 1e15f, 1e15f, 0.f);
+      if (ce < k1) {
+        const ChunkEnt K = klist[ce];
+        if (off < K.count) {
+          const float4 s = src32[K.start + off];
+          r = make_float4(s.x + K.sx, s.y + K.sy, s.z + K.sz, s.w);
         }
       }
-    }
+      return r;
+    };
+    float4 nxt = load(k0);
+    int buf = 0;
+    for (int64_t e = k0; e < k1; e += CPT) {
+      tile[buf][0][lane] = nxt;
+      __syncthreads();
+      nxt = load(e + CPT);
+      const int nslots = (int)((k1 - e < CPT ? k1 - e : CPT) * CH);
+#pragma unroll 4
+      for (int k = 0; k < nslots; k++) {
+        const float4 s = tile[buf][0][k];
 #pragma unroll
-    for (int q = 0; q < TPL; q++) {
-      if (KAHAN) {
-        kahan_add(px[q], sx[q], cx[q]);
-        kahan_add(py[q], sy[q], cy[q]);
-        kahan_add(pz[q], sz[q], cz[q]);
-      } else {
-        sx[q] += px[q];
-        sy[q] += py[q];
-        sz[q] += pz[q];
+        for (int q = 0; q < TPL; q++) p2p<ZEPS>(s, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
       }
+      flush<TPL, KAHAN>(px, py, pz, sx, sy, sz, cx, cy, cz);
+      buf ^= 1;
     }
   }
 #pragma unroll
@@ -361,12 +514,38 @@ __global__ __launch_bounds__(64) void k_group_eval(int64_t m, int gs, const doub
 }
 
 // ---------------------------------------------------------------- host
+template <int TPL, int CH>
+static void launch_eval(bh_ctx* c, int64_t ng, bool zeps, bool self, bool kahan, bool quad, int64_t m, int gs,
+                        const double4* tpos, const uint32_t* torig, const bh_tree* t, const GroupBox* boxes,
+                        const int64_t* coff, const CellEnt* clist, const int64_t* koff, const ChunkEnt* klist,
+                        float eps2, double G, double* out) {
+#define BH_EV(Z, S, K, Q)                                                                                       \
+  hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3((unsigned)ng), dim3(64), 0, c->stream, m, gs, tpos, \
+                     torig, t->spart32, t->pleaf, t->center, t->dim, boxes, coff, clist, koff, klist, eps2, G, out)
+#define BH_EV_Q(Z, S, K) \
+  if (quad) BH_EV(Z, S, K, true); else BH_EV(Z, S, K, false)
+#define BH_EV_K(Z, S) \
+  if (kahan) { BH_EV_Q(Z, S, true); } else { BH_EV_Q(Z, S, false); }
+#define BH_EV_S(Z) \
+  if (self) { BH_EV_K(Z, true); } else { BH_EV_K(Z, false); }
+  if (zeps) {
+    BH_EV_S(true);
+  } else {
+    BH_EV_S(false);
+  }
+#undef BH_EV_S
+#undef BH_EV_K
+#undef BH_EV_Q
+#undef BH_EV
+}
+
 int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, double eps, double G,
              const bh_f32_opts* o, double* out, int64_t* stats) {
   if (m == 0) return BH_OK;
   const int gs = o && o->group_size ? o->group_size : 64;
   const bool kahan = o ? o->kahan != 0 : true;
   const int stack_cap = o && o->stack_cap ? o->stack_cap : 2048;
+  const int multipole = o && o->multipole ? o->multipole : 2;
   if (gs != 64 && gs != 128) {
     set_error("group_size must be 64 or 128, got %d", gs);
     return BH_EINVAL;
@@ -375,10 +554,17 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     set_error("stack_cap must be in [64, 8192], got %d", stack_cap);
     return BH_EINVAL;
   }
+  if (multipole != 1 && multipole != 2) {
+    set_error("multipole must be 1 (monopole) or 2 (quadrupole), got %d", multipole);
+    return BH_EINVAL;
+  }
   if (!t->f32_ready) {
+    ProfScope p(c, "f32_prepare");
     int rc = f32_prepare(c, t);
     if (rc) return rc;
   }
+  const bool quad = multipole == 2 && t->quad_ready;
+  const int ch = t->leaf <= 16 ? 16 : (t->leaf <= 32 ? 32 : 64);
   Scratch s(c);
   const double4* tpos;
   const uint32_t* torig;
@@ -410,13 +596,13 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   {
     ProfScope p(c, "k_group_walk_count");
     if (t->dim == 3)
-      hipLaunchKernelGGL((k_group_walk<3, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes,
-                         t->center, ng, boxes, theta2, stack_cap, cnt, cnt + ng, cnt + 2 * ng, nullptr, nullptr,
-                         nullptr, nullptr, err);
+      hipLaunchKernelGGL((k_group_walk<3, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
+                         t->quad, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
+                         nullptr, nullptr, nullptr, err);
     else
-      hipLaunchKernelGGL((k_group_walk<2, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes,
-                         t->center, ng, boxes, theta2, stack_cap, cnt, cnt + ng, cnt + 2 * ng, nullptr, nullptr,
-                         nullptr, nullptr, err);
+      hipLaunchKernelGGL((k_group_walk<2, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
+                         t->quad, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
+                         nullptr, nullptr, nullptr, err);
     BH_LAUNCH_CHECK();
   }
   std::vector<int64_t> hc(3 * ng);
@@ -429,32 +615,32 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     return BH_EOVERFLOW;
   }
   std::vector<int64_t> ho(2 * (ng + 1));
-  int64_t mono_int = 0, pair_int = 0;
+  int64_t cell_int = 0, pair_int = 0;
   ho[0] = 0;
   ho[ng + 1] = 0;
   for (int64_t g = 0; g < ng; g++) {
     ho[g + 1] = ho[g] + hc[g];
     ho[ng + 1 + g + 1] = ho[ng + 1 + g] + hc[ng + g];
     const int64_t tg = std::min<int64_t>(gs, m - g * gs);
-    mono_int += hc[g] * tg;
+    cell_int += hc[g] * tg;
     pair_int += hc[2 * ng + g] * tg;
   }
-  if (!tgt) pair_int -= m;  // each target skips itself once in its own leaf
-  const int64_t ntm = ho[ng], ntl = ho[2 * ng + 1];
+  if (!tgt) pair_int -= m;  // each target's own pair contributes nothing
+  const int64_t ntc = ho[ng], ntk = ho[2 * ng + 1];
   BH_HIP(hipMemcpyAsync(offs, ho.data(), 2 * (ng + 1) * 8, hipMemcpyHostToDevice, c->stream));
-  float4* mlist = s.get<float4>((size_t)std::max<int64_t>(ntm, 1));
-  LeafEnt* llist = s.get<LeafEnt>((size_t)std::max<int64_t>(ntl, 1));
-  if (!mlist || !llist) return BH_ENOMEM;
+  CellEnt* clist = s.get<CellEnt>((size_t)std::max<int64_t>(ntc, 1));
+  ChunkEnt* klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(ntk, 1));
+  if (!clist || !klist) return BH_ENOMEM;
   {
     ProfScope p(c, "k_group_walk_fill");
     if (t->dim == 3)
       hipLaunchKernelGGL((k_group_walk<3, true>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
-                         ng, boxes, theta2, stack_cap, nullptr, nullptr, nullptr, offs, offs + ng + 1, mlist, llist,
-                         err);
+                         t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr, offs,
+                         offs + ng + 1, clist, klist, err);
     else
       hipLaunchKernelGGL((k_group_walk<2, true>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
-                         ng, boxes, theta2, stack_cap, nullptr, nullptr, nullptr, offs, offs + ng + 1, mlist, llist,
-                         err);
+                         t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr, offs,
+                         offs + ng + 1, clist, klist, err);
     BH_LAUNCH_CHECK();
   }
   const float eps2 = (float)(eps * eps);
@@ -462,31 +648,24 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const bool self = tgt == nullptr;
   {
     ProfScope p(c, "k_group_eval_f32");
-#define BH_EV(TPL, Z, S, K)                                                                                  \
-  hipLaunchKernelGGL((k_group_eval<TPL, Z, S, K>), dim3((unsigned)ng), dim3(64), 0, c->stream, m, gs, tpos, torig, \
-                     boxes, offs, mlist, offs + ng + 1, llist, t->spart32, eps2, G, t->dim, out)
-#define BH_EV_K(TPL, Z, S) \
-  if (kahan) BH_EV(TPL, Z, S, true); else BH_EV(TPL, Z, S, false)
-#define BH_EV_S(TPL, Z) \
-  if (self) { BH_EV_K(TPL, Z, true); } else { BH_EV_K(TPL, Z, false); }
-#define BH_EV_Z(TPL) \
-  if (zeps) { BH_EV_S(TPL, true); } else { BH_EV_S(TPL, false); }
+    const int64_t* coff = offs;
+    const int64_t* koff = offs + ng + 1;
     if (gs == 64) {
-      BH_EV_Z(1);
+      if (ch == 16) launch_eval<1, 16>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      else if (ch == 32) launch_eval<1, 32>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      else launch_eval<1, 64>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
     } else {
-      BH_EV_Z(2);
+      if (ch == 16) launch_eval<2, 16>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      else if (ch == 32) launch_eval<2, 32>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      else launch_eval<2, 64>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
     }
-#undef BH_EV_Z
-#undef BH_EV_S
-#undef BH_EV_K
-#undef BH_EV
     BH_LAUNCH_CHECK();
   }
   if (stats) {
-    stats[0] = mono_int;
+    stats[0] = cell_int;
     stats[1] = pair_int;
     stats[2] = ng;
-    stats[3] = ntm + ntl;
+    stats[3] = ntc + ntk;
   }
   return BH_OK;
 }

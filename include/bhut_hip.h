@@ -34,12 +34,13 @@ extern "C" {
 typedef struct bh_ctx bh_ctx;   /* device, stream, cached allocations */
 typedef struct bh_tree bh_tree; /* a tree resident in device memory   */
 
-/* Options of the FP32 relative-coordinate, compensated, grouped form. */
+/* Options of the FP32 relative-coordinate, compensated, grouped form.
+ * Zero fields take the defaults in brackets. */
 typedef struct {
-  int32_t group_size; /* targets per group: 64 (one per lane) or 128 (two)  */
-  int32_t kahan;      /* 1: Kahan-compensated per-target sums               */
-  int32_t stack_cap;  /* LDS stack entries of the group walk (<= 4096)      */
-  int32_t reserved;
+  int32_t group_size; /* targets per group: 64 (one per lane) or 128 [64]   */
+  int32_t kahan;      /* 1: Kahan-compensated per-target sums, 0: plain     */
+  int32_t stack_cap;  /* LDS stack entries of the group walk, <= 8192 [2048] */
+  int32_t multipole;  /* accepted cells: 1 monopole, 2 quadrupole [2]       */
 } bh_f32_opts;
 
 /* ---------------------------------------------------------------- context */

@@ -37,21 +37,24 @@ def test_theta_zero_is_fp32_direct_sum(native):
     assert rms < 2e-6 and mx < 2e-5, (rms, mx)
 
 
-@pytest.mark.parametrize("gs,kahan", [(64, 1), (128, 1), (64, 0)])
-def test_theta_half_accuracy(native, gs, kahan):
+@pytest.mark.parametrize("gs,kahan,multipole", [(64, 1, 2), (128, 1, 2), (64, 0, 2), (64, 1, 1)])
+def test_theta_half_accuracy(native, gs, kahan, multipole):
+    """rms relative error against FP64 direct summation: with quadrupole
+    cells below 1e-3 at theta = 0.5; monopole-only no worse than the plain
+    per-target walk (the group test only ever opens more)."""
     n = 100_000
     x, m = workloads.plummer(n, seed=1)
     dt = native.DeviceTree.build(x, m, 32)
-    opts = native.F32Opts(gs, kahan, 0, 0)
+    opts = native.F32Opts(gs, kahan, 0, multipole)
     a32, st = dt.accel_f32(None, 0.5, 0.01, 1.0, opts, stats=True)
     a64 = dt.accel_self(0.5, 0.01, 1.0)
     sel = np.random.default_rng(0).choice(n, 2000, replace=False)
     d = native.direct(x, m, x[sel], softening=0.01)
     rms32, _ = rel_err(a32[sel], d)
     rms64, _ = rel_err(a64[sel], d)
-    # the group test only ever opens more than the per-target test
     assert rms32 <= rms64 * 1.05, (rms32, rms64)
-    assert rms32 < 3e-3
+    if multipole == 2:
+        assert rms32 < 1e-3, rms32
     assert st[0] > 0 and st[1] > 0 and st[2] == -(-n // gs)
 
 
