@@ -53,6 +53,10 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-plain", action="store_true", help="skip timing the plain form beside the FP32 form")
+    p.add_argument("--group-size", type=int, default=0, help="FP32 form targets per group (0: autotuned)")
+    p.add_argument("--stack-cap", type=int, default=0, help="FP32 form LDS stack entries (0: autotuned)")
+    p.add_argument("--multipole", type=int, default=2, help="FP32 form cells: 1 monopole, 2 quadrupole")
+    p.add_argument("--no-kahan", action="store_true")
     return p.parse_args()
 
 
@@ -117,7 +121,21 @@ def main():
     out_t = torch.empty((N, 3), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
-    f32_opts = None
+    tuning = None
+    if args.form == "fp32" and (args.group_size == 0 or args.stack_cap == 0):
+        from bhut_amd import autotune
+        log("bench: autotuning the FP32 form (group size, stack depth) on this input")
+        probe = _native.DeviceTree.build_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, leaf, context=ctx)
+        best, table = autotune.tune_f32(probe, theta, eps, G, multipole=args.multipole,
+                                        kahan=not args.no_kahan,
+                                        group_sizes=(args.group_size,) if args.group_size else (64, 128),
+                                        stack_caps=(args.stack_cap,) if args.stack_cap else (1024, 2048))
+        del probe
+        tuning = {"chosen": best, "table": table}
+        log(f"bench: autotune -> {best}")
+        f32_opts = _native.F32Opts(best["group_size"], 0 if args.no_kahan else 1, best["stack_cap"], args.multipole)
+    else:
+        f32_opts = _native.F32Opts(args.group_size, 0 if args.no_kahan else 1, args.stack_cap, args.multipole)
 
     def step(form):
         t = _native.DeviceTree.build_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, leaf, context=ctx)
@@ -224,6 +242,9 @@ def main():
         "interactions_per_s": inter / (ms * 1e-3),
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in res[head]["prof"].items()},
         "gpu": torch.cuda.get_device_name(0),
+        "f32_options": None if head != "fp32" else {
+            "group_size": f32_opts.group_size, "stack_cap": f32_opts.stack_cap, "kahan": f32_opts.kahan,
+            "multipole": f32_opts.multipole, "autotune": tuning},
     }
     if head == "fp32" and "fp64" in res:
         line["plain_fp64"] = {"value": N / (res["fp64"]["ms"] * 1e-3), "ms_per_step": res["fp64"]["ms"],

@@ -1,0 +1,78 @@
+"""Autotuner for the FP32 grouped form.
+
+Times the group walk and evaluation kernels (HIP events recorded by the
+library on its stream) for each candidate configuration on the caller's own
+tree and input, and returns the fastest.  Candidates whose LDS stack is too
+small for the tree (the walk reports an overflow) are skipped.  Leaf size
+changes the tree itself, so `tune_leaf_f32` rebuilds per candidate and also
+times the build.
+"""
+
+from __future__ import annotations
+
+from typing import Iterable
+
+import numpy as np
+
+from . import _native
+
+_F32_KERNELS = ("k_group_walk_count", "k_group_walk_fill", "k_group_eval_f32", "f32_prepare")
+
+
+def _timed(ctx, fn, reps: int) -> float:
+    fn()  # warm-up (also builds FP32 source copies and moments)
+    ctx.profile(True)
+    for _ in range(reps):
+        fn()
+    prof = ctx.profile_summary()
+    ctx.profile(False)
+    return sum(ms for name, (_, ms) in prof.items() if name in _F32_KERNELS or name.startswith("build_")) / reps
+
+
+def tune_f32(tree: "_native.DeviceTree", theta: float, softening: float, G: float = 1.0, *, targets=None,
+             group_sizes: Iterable[int] = (64, 128), stack_caps: Iterable[int] = (1024, 2048), multipole: int = 2,
+             kahan: bool = True, reps: int = 2):
+    """Best (group_size, stack_cap) for this tree: returns (best, table)."""
+    ctx = tree.context
+    table = []
+    for gs in group_sizes:
+        for sc in stack_caps:
+            opts = _native.F32Opts(int(gs), int(kahan), int(sc), int(multipole))
+            try:
+                ms = _timed(ctx, lambda: tree.accel_f32(targets, theta, softening, G, opts), reps)
+            except RuntimeError as e:
+                if "stack" not in str(e):
+                    raise
+                table.append({"group_size": gs, "stack_cap": sc, "ms": None, "note": "stack overflow"})
+                continue
+            table.append({"group_size": gs, "stack_cap": sc, "ms": ms})
+    ok = [r for r in table if r["ms"] is not None]
+    if not ok:
+        raise RuntimeError("no FP32 configuration fits; raise the stack candidates")
+    best = min(ok, key=lambda r: r["ms"])
+    return {"group_size": best["group_size"], "stack_cap": best["stack_cap"]}, table
+
+
+def tune_leaf_f32(positions, masses, theta: float, softening: float, G: float = 1.0, *,
+                  leaf_sizes: Iterable[int] = (16, 32, 64), group_sizes: Iterable[int] = (64, 128),
+                  stack_caps: Iterable[int] = (2048,), multipole: int = 2, reps: int = 1, context=None):
+    """Best (leaf_size, group_size, stack_cap) including the tree build."""
+    ctx = context or _native.ctx()
+    pos = np.ascontiguousarray(positions, dtype=np.float64)
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    table = []
+    for leaf in leaf_sizes:
+        ctx.profile(True)
+        tree = _native.DeviceTree.build(pos, m, int(leaf), context=ctx)
+        prof = ctx.profile_summary()
+        ctx.profile(False)
+        build_ms = sum(ms for name, (_, ms) in prof.items() if name.startswith("build_"))
+        best, sub = tune_f32(tree, theta, softening, G, group_sizes=group_sizes, stack_caps=stack_caps,
+                             multipole=multipole, reps=reps)
+        for r in sub:
+            table.append(dict(r, leaf_size=leaf, build_ms=build_ms,
+                              total_ms=None if r["ms"] is None else r["ms"] + build_ms))
+        del tree
+    ok = [r for r in table if r["total_ms"] is not None]
+    best = min(ok, key=lambda r: r["total_ms"])
+    return {"leaf_size": best["leaf_size"], "group_size": best["group_size"], "stack_cap": best["stack_cap"]}, table

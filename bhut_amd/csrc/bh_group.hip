@@ -400,22 +400,31 @@ __device__ __forceinline__ void flush(float (&px)[TPL], float (&py)[TPL], float 
   }
 }
 
+// Each wavefront reads only the LDS tile it wrote itself, and a wavefront's
+// LDS operations complete in issue order, so a compiler-level wave barrier
+// orders the tile hand-off; no workgroup barrier is needed.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int EV_WAVES = 4;  // groups (wavefronts) per workgroup
+
 // CH: particles per chunk (64/CH chunks fill one 64-slot tile).
 template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
-__global__ __launch_bounds__(64) void k_group_eval(int64_t m, int gs, const double4* __restrict__ tpos,
-                                                   const uint32_t* __restrict__ torig,
-                                                   const float4* __restrict__ src32,
-                                                   const int32_t* __restrict__ pleaf,
-                                                   const double* __restrict__ center, int dim,
-                                                   const GroupBox* __restrict__ boxes,
-                                                   const int64_t* __restrict__ coff,
-                                                   const CellEnt* __restrict__ clist,
-                                                   const int64_t* __restrict__ koff,
-                                                   const ChunkEnt* __restrict__ klist, float eps2, double G,
-                                                   double* __restrict__ out) {
-  __shared__ float4 tile[2][3][64];
-  const int64_t g = blockIdx.x;
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(64 * EV_WAVES) void k_group_eval(
+    int64_t ngroups, int64_t m, int gs, const double4* __restrict__ tpos, const uint32_t* __restrict__ torig,
+    const float4* __restrict__ src32, const int32_t* __restrict__ pleaf, const double* __restrict__ center, int dim,
+    const GroupBox* __restrict__ boxes, const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist,
+    const int64_t* __restrict__ koff, const ChunkEnt* __restrict__ klist, float eps2, double G,
+    double* __restrict__ out) {
+  __shared__ float4 tiles[EV_WAVES][2][3][64];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int64_t g = (int64_t)blockIdx.x * EV_WAVES + w;
+  if (g >= ngroups) return;  // whole wavefront
+  float4 (*tile)[3][64] = tiles[w];
+  const int lane = threadIdx.x % 64;
   const int64_t t0 = g * gs;
   const int64_t t1 = t0 + gs < m ? t0 + gs : m;
   const GroupBox B = boxes[g];
@@ -454,7 +463,7 @@ __global__ __launch_bounds__(64) void k_group_eval(int64_t m, int gs, const doub
       tile[buf][0][lane] = nxt.a;
       tile[buf][1][lane] = nxt.b;
       tile[buf][2][lane] = nxt.c;
-      __syncthreads();
+      wave_sync();
       if (e + 64 + lane < c1) nxt = clist[e + 64 + lane];
       const int cnt = (int)(c1 - e < 64 ? c1 - e : 64);
       for (int k = 0; k < cnt; k++) {
@@ -489,7 +498,7 @@ __global__ __launch_bounds__(64) void k_group_eval(int64_t m, int gs, const doub
     int buf = 0;
     for (int64_t e = k0; e < k1; e += CPT) {
       tile[buf][0][lane] = nxt;
-      __syncthreads();
+      wave_sync();
       nxt = load(e + CPT);
       const int nslots = (int)((k1 - e < CPT ? k1 - e : CPT) * CH);
 #pragma unroll 4
@@ -520,8 +529,9 @@ static void launch_eval(bh_ctx* c, int64_t ng, bool zeps, bool self, bool kahan,
                         const int64_t* coff, const CellEnt* clist, const int64_t* koff, const ChunkEnt* klist,
                         float eps2, double G, double* out) {
 #define BH_EV(Z, S, K, Q)                                                                                       \
-  hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3((unsigned)ng), dim3(64), 0, c->stream, m, gs, tpos, \
-                     torig, t->spart32, t->pleaf, t->center, t->dim, boxes, coff, clist, koff, klist, eps2, G, out)
+  hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3((unsigned)cdiv(ng, EV_WAVES)), dim3(64 * EV_WAVES), 0, \
+                     c->stream, ng, m, gs, tpos, torig, t->spart32, t->pleaf, t->center, t->dim, boxes, coff, clist,   \
+                     koff, klist, eps2, G, out)
 #define BH_EV_Q(Z, S, K) \
   if (quad) BH_EV(Z, S, K, true); else BH_EV(Z, S, K, false)
 #define BH_EV_K(Z, S) \
