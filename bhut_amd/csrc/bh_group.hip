@@ -412,18 +412,23 @@ __device__ __forceinline__ void wave_sync() {
 constexpr int EV_WAVES = 4;  // groups (wavefronts) per workgroup
 
 // CH: particles per chunk (64/CH chunks fill one 64-slot tile).
+// One LDS tile per wavefront, refilled in place: the wavefront issued every
+// read of the old tile before its writes of the new one, and its LDS
+// operations execute in order.  The next tile's HBM reads are in flight in
+// registers while the current tile is consumed.
 template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
-__global__ __launch_bounds__(64 * EV_WAVES) void k_group_eval(
-    int64_t ngroups, int64_t m, int gs, const double4* __restrict__ tpos, const uint32_t* __restrict__ torig,
-    const float4* __restrict__ src32, const int32_t* __restrict__ pleaf, const double* __restrict__ center, int dim,
-    const GroupBox* __restrict__ boxes, const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist,
-    const int64_t* __restrict__ koff, const ChunkEnt* __restrict__ klist, float eps2, double G,
-    double* __restrict__ out) {
-  __shared__ float4 tiles[EV_WAVES][2][3][64];
+__global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
+    int64_t ngroups, const int32_t* __restrict__ order, int64_t m, int gs, const double4* __restrict__ tpos,
+    const uint32_t* __restrict__ torig, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
+    const double* __restrict__ center, int dim, const GroupBox* __restrict__ boxes,
+    const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
+    const ChunkEnt* __restrict__ klist, float eps2, double G, double* __restrict__ out) {
+  __shared__ float4 tiles[EV_WAVES][3][64];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-  const int64_t g = (int64_t)blockIdx.x * EV_WAVES + w;
-  if (g >= ngroups) return;  // whole wavefront
-  float4 (*tile)[3][64] = tiles[w];
+  const int64_t slot = (int64_t)blockIdx.x * EV_WAVES + w;
+  if (slot >= ngroups) return;  // whole wavefront
+  const int64_t g = order ? (int64_t)order[slot] : slot;
+  float4 (*tile)[64] = tiles[w];
   const int lane = threadIdx.x % 64;
   const int64_t t0 = g * gs;
   const int64_t t1 = t0 + gs < m ? t0 + gs : m;
@@ -455,24 +460,31 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_group_eval(
   // ---- accepted cells: tiles of 64 entries
   const int64_t c0 = coff[g], c1 = coff[g + 1];
   if (c1 > c0) {
-    CellEnt nxt;
-    int64_t e = c0;
-    if (e + lane < c1) nxt = clist[e + lane];
-    int buf = 0;
-    for (; e < c1; e += 64) {
-      tile[buf][0][lane] = nxt.a;
-      tile[buf][1][lane] = nxt.b;
-      tile[buf][2][lane] = nxt.c;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 na = z4, nb = z4, nc = z4;
+    if (c0 + lane < c1) {
+      na = clist[c0 + lane].a;
+      nb = clist[c0 + lane].b;
+      nc = clist[c0 + lane].c;
+    }
+    for (int64_t e = c0; e < c1; e += 64) {
       wave_sync();
-      if (e + 64 + lane < c1) nxt = clist[e + 64 + lane];
+      tile[0][lane] = na;
+      tile[1][lane] = nb;
+      tile[2][lane] = nc;
+      wave_sync();
+      if (e + 64 + lane < c1) {
+        na = clist[e + 64 + lane].a;
+        nb = clist[e + 64 + lane].b;
+        nc = clist[e + 64 + lane].c;
+      }
       const int cnt = (int)(c1 - e < 64 ? c1 - e : 64);
       for (int k = 0; k < cnt; k++) {
-        const float4 a = tile[buf][0][k], b = tile[buf][1][k], c = tile[buf][2][k];
+        const float4 a = tile[0][k], b = tile[1][k], c = tile[2][k];
 #pragma unroll
         for (int q = 0; q < TPL; q++) cell<ZEPS, QUAD>(a, b, c, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
       }
       flush<TPL, KAHAN>(px, py, pz, sx, sy, sz, cx, cy, cz);
-      buf ^= 1;
     }
   }
 
@@ -495,20 +507,19 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_group_eval(
       return r;
     };
     float4 nxt = load(k0);
-    int buf = 0;
     for (int64_t e = k0; e < k1; e += CPT) {
-      tile[buf][0][lane] = nxt;
+      wave_sync();
+      tile[0][lane] = nxt;
       wave_sync();
       nxt = load(e + CPT);
       const int nslots = (int)((k1 - e < CPT ? k1 - e : CPT) * CH);
 #pragma unroll 4
       for (int k = 0; k < nslots; k++) {
-        const float4 s = tile[buf][0][k];
+        const float4 s = tile[0][k];
 #pragma unroll
         for (int q = 0; q < TPL; q++) p2p<ZEPS>(s, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
       }
       flush<TPL, KAHAN>(px, py, pz, sx, sy, sz, cx, cy, cz);
-      buf ^= 1;
     }
   }
 #pragma unroll
@@ -522,16 +533,28 @@ __global__ __launch_bounds__(64 * EV_WAVES) void k_group_eval(
   }
 }
 
+// Longest-first schedule: key = complement of the group's estimated work
+// (particle pairs + 3 x accepted cells) so an ascending sort puts the most
+// expensive groups at the front of the dispatch order.
+__global__ void k_group_cost_keys(int64_t ng, const int64_t* __restrict__ ccount, const int64_t* __restrict__ pcount,
+                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const uint64_t cost = (uint64_t)pcount[g] + 3ull * (uint64_t)ccount[g];
+  keys[g] = 0xffffffffull - (cost < 0xffffffffull ? cost : 0xffffffffull);
+  idx[g] = (uint32_t)g;
+}
+
 // ---------------------------------------------------------------- host
 template <int TPL, int CH>
-static void launch_eval(bh_ctx* c, int64_t ng, bool zeps, bool self, bool kahan, bool quad, int64_t m, int gs,
-                        const double4* tpos, const uint32_t* torig, const bh_tree* t, const GroupBox* boxes,
-                        const int64_t* coff, const CellEnt* clist, const int64_t* koff, const ChunkEnt* klist,
-                        float eps2, double G, double* out) {
+static void launch_eval(bh_ctx* c, int64_t ng, const int32_t* order, bool zeps, bool self, bool kahan, bool quad,
+                        int64_t m, int gs, const double4* tpos, const uint32_t* torig, const bh_tree* t,
+                        const GroupBox* boxes, const int64_t* coff, const CellEnt* clist, const int64_t* koff,
+                        const ChunkEnt* klist, float eps2, double G, double* out) {
 #define BH_EV(Z, S, K, Q)                                                                                       \
   hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3((unsigned)cdiv(ng, EV_WAVES)), dim3(64 * EV_WAVES), 0, \
-                     c->stream, ng, m, gs, tpos, torig, t->spart32, t->pleaf, t->center, t->dim, boxes, coff, clist,   \
-                     koff, klist, eps2, G, out)
+                     c->stream, ng, order, m, gs, tpos, torig, t->spart32, t->pleaf, t->center, t->dim, boxes, coff,  \
+                     clist, koff, klist, eps2, G, out)
 #define BH_EV_Q(Z, S, K) \
   if (quad) BH_EV(Z, S, K, true); else BH_EV(Z, S, K, false)
 #define BH_EV_K(Z, S) \
@@ -656,18 +679,34 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const float eps2 = (float)(eps * eps);
   const bool zeps = !(eps2 > 0.f);
   const bool self = tgt == nullptr;
+  const int32_t* order = nullptr;
+  if (!getenv("BHUT_EVAL_INDEX_ORDER")) {
+    ProfScope p(c, "k_group_order");
+    uint64_t* keys = s.get<uint64_t>(ng);
+    uint64_t* keys2 = s.get<uint64_t>(ng);
+    uint32_t* idx = s.get<uint32_t>(ng);
+    uint32_t* idx2 = s.get<uint32_t>(ng);
+    if (!keys || !keys2 || !idx || !idx2) return BH_ENOMEM;
+    hipLaunchKernelGGL(k_group_cost_keys, dim3((unsigned)cdiv(ng, 256)), dim3(256), 0, c->stream, ng, cnt,
+                       cnt + 2 * ng, keys, idx);
+    uint64_t* ko;
+    uint32_t* vo;
+    rc = radix_sort(c, keys, idx, ng, 32, &ko, &vo, keys2, idx2);
+    if (rc) return rc;
+    order = reinterpret_cast<const int32_t*>(vo);
+  }
   {
     ProfScope p(c, "k_group_eval_f32");
     const int64_t* coff = offs;
     const int64_t* koff = offs + ng + 1;
     if (gs == 64) {
-      if (ch == 16) launch_eval<1, 16>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
-      else if (ch == 32) launch_eval<1, 32>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
-      else launch_eval<1, 64>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      if (ch == 16) launch_eval<1, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      else if (ch == 32) launch_eval<1, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      else launch_eval<1, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
     } else {
-      if (ch == 16) launch_eval<2, 16>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
-      else if (ch == 32) launch_eval<2, 32>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
-      else launch_eval<2, 64>(c, ng, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      if (ch == 16) launch_eval<2, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      else if (ch == 32) launch_eval<2, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      else launch_eval<2, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
     }
     BH_LAUNCH_CHECK();
   }
