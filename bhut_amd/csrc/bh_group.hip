@@ -493,25 +493,47 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
   constexpr int CPT = 64 / CH;
   const int64_t k0 = koff[g], k1 = koff[g + 1];
   if (k1 > k0) {
+    // Two-stage pipeline over tiles: the chunk entries of tile i+2 and the
+    // sources of tile i+1 are in flight while tile i is consumed; the shift
+    // is applied only when a tile is written to LDS, so no load result is
+    // needed before its tile comes up.
     const int sub = lane / CH, off = lane % CH;
-    auto load = [&](int64_t e) -> float4 {
+    const float4* kl4 = reinterpret_cast<const float4*>(klist);  // ChunkEnt = 2 x 16 B
+    // raw chunk fields: shift xyz + start bits in .w, count bits in .x
+    auto load_chunk = [&](int64_t e, float4& ka, float4& kb) {
       const int64_t ce = e + sub;
-      float4 r = make_float4(1e15f, 1e15f, 1e15f, 0.f);
       if (ce < k1) {
-        const ChunkEnt K = klist[ce];
-        if (off < K.count) {
-          const float4 s = src32[K.start + off];
-          r = make_float4(s.x + K.sx, s.y + K.sy, s.z + K.sz, s.w);
-        }
+        ka = kl4[2 * ce];
+        kb = kl4[2 * ce + 1];
+      } else {
+        ka = make_float4(0.f, 0.f, 0.f, 0.f);
+        kb = make_float4(0.f, 0.f, 0.f, 0.f);  // count 0: every slot is padding
       }
-      return r;
     };
-    float4 nxt = load(k0);
+    auto src_index = [&](const float4& ka, const float4& kb) -> int {
+      return off < __float_as_int(kb.x) ? __float_as_int(ka.w) + off : -1;
+    };
+    auto load_src = [&](int kidx) -> float4 {
+      float4 s = make_float4(1e15f, 1e15f, 1e15f, 0.f);
+      if (kidx >= 0) s = src32[kidx];
+      return s;
+    };
+    float4 ka_cur, kb_cur, ka_nxt, kb_nxt;
+    load_chunk(k0, ka_cur, kb_cur);
+    int idx_cur = src_index(ka_cur, kb_cur);
+    float4 src_cur = load_src(idx_cur);
+    load_chunk(k0 + CPT, ka_nxt, kb_nxt);
     for (int64_t e = k0; e < k1; e += CPT) {
       wave_sync();
-      tile[0][lane] = nxt;
+      tile[0][lane] = idx_cur >= 0 ? make_float4(src_cur.x + ka_cur.x, src_cur.y + ka_cur.y,
+                                                 src_cur.z + ka_cur.z, src_cur.w)
+                                   : make_float4(1e15f, 1e15f, 1e15f, 0.f);
       wave_sync();
-      nxt = load(e + CPT);
+      ka_cur = ka_nxt;
+      kb_cur = kb_nxt;
+      idx_cur = src_index(ka_cur, kb_cur);
+      src_cur = load_src(idx_cur);
+      load_chunk(e + 2 * CPT, ka_nxt, kb_nxt);
       const int nslots = (int)((k1 - e < CPT ? k1 - e : CPT) * CH);
 #pragma unroll 4
       for (int k = 0; k < nslots; k++) {
