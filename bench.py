@@ -218,6 +218,12 @@ def main():
     launches, kms = res[head]["prof"].get(kname, (0, 0.0))
     kavg = kms / max(launches, 1)
     achieved = flops / (kavg * 1e-3) / 1e12 if kavg > 0 else None
+    # HBM bytes per launch come from the committed FETCH_SIZE/WRITE_SIZE
+    # passes (rocprofv3) of this same workload, when it matches
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "plummer1m_group_eval_traffic.json")
+    if head == "fp32" and N == 1_000_000 and leaf == 32 and f32_opts.group_size in (0, 64) and os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": N / (ms * 1e-3),
@@ -235,7 +241,7 @@ def main():
                    "leaf_size": leaf, "G": G, "seed": 1, "step": "tree build + self accelerations",
                    "form": head},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": (achieved / peak) if achieved else None, "traffic": None,
+                     "frac": (achieved / peak) if achieved else None, "traffic": traffic,
                      "kernel": kname, "kernel_avg_ms": kavg,
                      "flops_per_launch": flops, "interactions_per_launch": inter},
         "accuracy": {f: {"rms_rel_err": acc_err[f][0], "max_rel_err": acc_err[f][1]} for f in acc_err},
