@@ -6,6 +6,7 @@ hand-written HIP kernels (libbhut_hip.so, C ABI in include/bhut_hip.h).
 """
 
 from .api import Tree, accelerations
+from .batched import Forest, accelerations_batched
 
 __version__ = "0.1.0"
-__all__ = ["Tree", "accelerations"]
+__all__ = ["Tree", "accelerations", "Forest", "accelerations_batched"]

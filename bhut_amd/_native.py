@@ -44,6 +44,9 @@ _SIGS = {
     "bh_tree_build_dev": (ctypes.c_int, [vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
     "bh_tree_import": (ctypes.c_int, [vp, i64, i64, i32, i32] + [vp] * 13 + [ctypes.POINTER(vp)]),
     "bh_tree_clone": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+    "bh_forest_build": (ctypes.c_int, [vp, vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
+    "bh_forest_build_dev": (ctypes.c_int, [vp, vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
+    "bh_tree_systems": (ctypes.c_int, [vp, ctypes.POINTER(i64), vp, vp]),
     "bh_tree_free": (None, [vp]),
     "bh_tree_shape": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
                                      ctypes.POINTER(i32), ctypes.POINTER(i32)]),
@@ -212,6 +215,38 @@ class DeviceTree:
                                     *[ptr(a) for a in arrs], *[ptr(a) for a in ints], ptr(pos), ptr(m),
                                     ctypes.byref(h)))
         return cls(h, c)
+
+    @classmethod
+    def build_forest(cls, positions: np.ndarray, masses: np.ndarray, counts, leaf_size: int,
+                     context: Context | None = None) -> "DeviceTree":
+        """One forest of len(counts) independent systems stored back to back."""
+        c = context or ctx()
+        pos, m = f64c(positions), f64c(masses)
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        if pos.ndim != 2 or m.shape != (pos.shape[0],) or int(cnt.sum()) != pos.shape[0]:
+            raise ValueError("positions (n, dim), masses (n,) and sum(counts) == n required")
+        h = vp()
+        check(load().bh_forest_build(c.handle, ptr(pos), ptr(m), ptr(cnt), len(cnt), pos.shape[1],
+                                     int(leaf_size), ctypes.byref(h)))
+        return cls(h, c)
+
+    @classmethod
+    def build_forest_dev(cls, pos_ptr: int, mass_ptr: int, counts, dim: int, leaf_size: int,
+                         context: Context | None = None) -> "DeviceTree":
+        c = context or ctx()
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        h = vp()
+        check(load().bh_forest_build_dev(c.handle, vp(pos_ptr), vp(mass_ptr), ptr(cnt), len(cnt), int(dim),
+                                         int(leaf_size), ctypes.byref(h)))
+        return cls(h, c)
+
+    def systems(self) -> tuple[np.ndarray, np.ndarray]:
+        """Particle and node offsets of the systems (nsys + 1 entries each)."""
+        ns = i64()
+        check(load().bh_tree_systems(self.handle, ctypes.byref(ns), None, None))
+        po, no = np.empty(ns.value + 1, np.int64), np.empty(ns.value + 1, np.int64)
+        check(load().bh_tree_systems(self.handle, ctypes.byref(ns), ptr(po), ptr(no)))
+        return po, no
 
     def clone(self) -> "DeviceTree":
         h = vp()

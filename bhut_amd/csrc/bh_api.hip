@@ -138,7 +138,7 @@ static void free_tree_arrays(bh_tree* t) {
   bh_ctx* c = t->ctx;
   void* ps[] = {t->spart, t->perm,  t->start, t->count, t->first_child, t->parent, t->skip,
                 t->leaf_order, t->center, t->half, t->mass, t->com, t->bmin, t->bmax, t->gbox,
-                t->nodes, t->spart32, t->pleaf, t->quad};
+                t->nodes, t->spart32, t->pleaf, t->quad, t->d_sys_poff, t->d_sys_noff};
   for (void* p : ps) dfree(c, p);
 }
 
@@ -162,14 +162,39 @@ static int alloc_topology(bh_tree* t) {
   return BH_OK;
 }
 
-static int build_dev(bh_ctx* c, const double* pos, const double* mass, int64_t n, int dim,
-                     int leaf, bh_tree* t) {
+// System offsets of a tree (nsys = 1) or forest, host and device copies.
+static int set_systems(bh_tree* t, const int64_t* counts, int64_t nsys) {
+  bh_ctx* c = t->ctx;
+  t->nsys = nsys;
+  t->sys_poff.assign(nsys + 1, 0);
+  for (int64_t s = 0; s < nsys; s++) t->sys_poff[s + 1] = t->sys_poff[s] + counts[s];
+  BH_TRY(forest_plan(counts, nsys, t->leaf, t->sys_noff, t->sys_loff, &t->M, &t->nleaves, &t->depth));
+  std::vector<int32_t> p32(nsys + 1), n32(nsys + 1);
+  for (int64_t s = 0; s <= nsys; s++) {
+    p32[s] = (int32_t)t->sys_poff[s];
+    n32[s] = (int32_t)t->sys_noff[s];
+  }
+  if (!alloc_into(c, t->d_sys_poff, (size_t)nsys + 1) || !alloc_into(c, t->d_sys_noff, (size_t)nsys + 1))
+    return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(t->d_sys_poff, p32.data(), (nsys + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  BH_HIP(hipMemcpyAsync(t->d_sys_noff, n32.data(), (nsys + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));  // the host vectors are temporaries
+  return BH_OK;
+}
+
+// build_tree for one system, or for a forest of independent systems whose
+// particles are consecutive runs of counts[s] (one segmented sort, one
+// topology launch, one upward pass for all of them).
+static int build_dev(bh_ctx* c, const double* pos, const double* mass, const int64_t* counts, int64_t nsys,
+                     int dim, int leaf, bh_tree* t) {
   t->ctx = c;
-  t->n = n;
   t->dim = dim;
   t->leaf = leaf;
+  int64_t n = 0;
+  for (int64_t s = 0; s < nsys; s++) n += counts[s];
+  t->n = n;
   Scratch s(c);
-  if (!alloc_into(c, t->gbox, 2 * (size_t)dim) || !alloc_into(c, t->spart, (size_t)n) ||
+  if (!alloc_into(c, t->gbox, 2 * (size_t)dim * nsys) || !alloc_into(c, t->spart, (size_t)n) ||
       !alloc_into(c, t->perm, (size_t)n))
     return BH_ENOMEM;
   uint64_t* k0 = s.get<uint64_t>(n);
@@ -178,7 +203,7 @@ static int build_dev(bh_ctx* c, const double* pos, const double* mass, int64_t n
   if (!k0 || !k1 || !v1) return BH_ENOMEM;
   uint32_t* v0 = reinterpret_cast<uint32_t*>(t->perm);
   // the (N, leaf) tables give M before the node arrays are allocated
-  BH_TRY(topology_plan(n, leaf, &t->M, &t->nleaves, &t->depth));
+  BH_TRY(set_systems(t, counts, nsys));
   t->binary = true;
   BH_TRY(alloc_topology(t));
   {
@@ -187,8 +212,13 @@ static int build_dev(bh_ctx* c, const double* pos, const double* mass, int64_t n
   }
   {
     ProfScope p(c, "build_bbox_morton");
-    BH_TRY(bbox(c, pos, n, dim, t->gbox));
-    BH_TRY(morton_iota(c, pos, n, dim, t->gbox, 0, k0, v0));
+    if (nsys == 1) {
+      BH_TRY(bbox(c, pos, n, dim, t->gbox));
+      BH_TRY(morton_iota(c, pos, n, dim, t->gbox, 0, k0, v0));
+    } else {
+      BH_TRY(bbox_seg(c, pos, t->d_sys_poff, nsys, dim, t->gbox));
+      BH_TRY(morton_seg(c, pos, n, dim, t->d_sys_poff, nsys, t->gbox, k0, v0));
+    }
   }
   {
     ProfScope p(c, "build_radix_sort");
@@ -196,6 +226,14 @@ static int build_dev(bh_ctx* c, const double* pos, const double* mass, int64_t n
     uint32_t* vo;
     BH_TRY(radix_sort(c, k0, v0, n, 21 * dim, &ko, &vo, k1, v1));
     if (vo != v0) BH_HIP(hipMemcpyAsync(v0, vo, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    if (nsys > 1) {
+      // stable passes on the system id: (system, Morton) order
+      BH_TRY(system_keys(c, v0, n, t->d_sys_poff, nsys, k0));
+      int bits = 0;
+      while ((1ll << bits) < nsys) bits++;
+      BH_TRY(radix_sort(c, k0, v0, n, bits, &ko, &vo, k1, v1));
+      if (vo != v0) BH_HIP(hipMemcpyAsync(v0, vo, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    }
   }
   {
     ProfScope p(c, "build_gather");
@@ -339,7 +377,8 @@ int bh_tree_build_dev(bh_ctx* c, const double* d_pos, const double* d_mass, int6
   BH_REQ(d_pos && d_mass, "NULL positions or masses");
   DeviceGuard g(c->device);
   bh_tree* t = new bh_tree();
-  int rc = build_dev(c, d_pos, d_mass, n, dim, leaf_size, t);
+  const int64_t counts[1] = {n};
+  int rc = build_dev(c, d_pos, d_mass, counts, 1, dim, leaf_size, t);
   if (rc == BH_OK) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) rc = hip_fail(e, "build", __FILE__, __LINE__);
@@ -371,6 +410,66 @@ int bh_tree_build(bh_ctx* c, const double* pos, const double* mass, int64_t n, i
   BH_HIP(hipMemcpyAsync(dp, pos, (size_t)n * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
   BH_HIP(hipMemcpyAsync(dm, mass, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   return bh_tree_build_dev(c, dp, dm, n, dim, leaf_size, out);
+}
+
+int bh_forest_build_dev(bh_ctx* c, const double* d_pos, const double* d_mass, const int64_t* counts,
+                        int64_t nsys, int32_t dim, int32_t leaf_size, bh_tree** out) {
+  BH_REQ(c && out && counts, "NULL argument");
+  *out = nullptr;
+  BH_REQ(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  BH_REQ(leaf_size > 0, "leaf_size must be positive, got %d", leaf_size);
+  BH_REQ(nsys >= 1 && nsys <= (1 << 24), "number of systems must be in [1, 2^24], got %lld", (long long)nsys);
+  int64_t n = 0;
+  for (int64_t s = 0; s < nsys; s++) {
+    if (counts[s] <= 0) {
+      set_error("Cannot build tree with zero particles (system %lld)", (long long)s);
+      return BH_EEMPTY;
+    }
+    n += counts[s];
+  }
+  BH_REQ(n < 0x7fffffffll, "n = %lld exceeds the int32 index range", (long long)n);
+  BH_REQ(d_pos && d_mass, "NULL positions or masses");
+  DeviceGuard g(c->device);
+  bh_tree* t = new bh_tree();
+  int rc = build_dev(c, d_pos, d_mass, counts, nsys, dim, leaf_size, t);
+  if (rc == BH_OK) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "forest build", __FILE__, __LINE__);
+  }
+  if (rc != BH_OK) {
+    free_tree_arrays(t);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return BH_OK;
+}
+
+int bh_forest_build(bh_ctx* c, const double* pos, const double* mass, const int64_t* counts, int64_t nsys,
+                    int32_t dim, int32_t leaf_size, bh_tree** out) {
+  BH_REQ(c && out && counts && pos && mass, "NULL argument");
+  *out = nullptr;
+  BH_REQ(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  int64_t n = 0;
+  for (int64_t s = 0; s < nsys; s++) n += counts[s] > 0 ? counts[s] : 0;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* dp = s.get<double>((size_t)n * dim);
+  double* dm = s.get<double>((size_t)n);
+  if (!dp || !dm) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(dp, pos, (size_t)n * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  BH_HIP(hipMemcpyAsync(dm, mass, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return bh_forest_build_dev(c, dp, dm, counts, nsys, dim, leaf_size, out);
+}
+
+int bh_tree_systems(const bh_tree* t, int64_t* nsys, int64_t* particle_offsets, int64_t* node_offsets) {
+  BH_REQ(t, "tree is NULL");
+  if (nsys) *nsys = t->nsys;
+  for (int64_t s = 0; s <= t->nsys; s++) {
+    if (particle_offsets) particle_offsets[s] = t->sys_poff[s];
+    if (node_offsets) node_offsets[s] = t->sys_noff[s];
+  }
+  return BH_OK;
 }
 
 void bh_tree_free(bh_tree* t) {
@@ -497,7 +596,12 @@ int bh_tree_import(bh_ctx* c, int64_t n, int64_t M, int32_t dim, int32_t leaf_si
   t->depth = 0;
   t->imported = true;
   t->binary = binary;
+  t->nsys = 1;
+  t->sys_poff = {0, n};
+  t->sys_noff = {0, M};
+  t->sys_loff = {0, (int64_t)leaves.size()};
   int rc = BH_OK;
+  const int32_t p32[2] = {0, (int32_t)n}, n32[2] = {0, (int32_t)M};
   std::vector<int32_t> st(M), cn(M), fc(M), pm(n);
   for (int64_t k = 0; k < M; k++) {
     st[k] = (int32_t)start[k];
@@ -515,8 +619,11 @@ int bh_tree_import(bh_ctx* c, int64_t n, int64_t M, int32_t dim, int32_t leaf_si
     }
   };
   if (!dp || !dm || !alloc_into(c, t->gbox, 2 * (size_t)dim) || !alloc_into(c, t->spart, (size_t)n) ||
-      !alloc_into(c, t->perm, (size_t)n) || alloc_topology(t) != BH_OK)
+      !alloc_into(c, t->perm, (size_t)n) || alloc_topology(t) != BH_OK || !alloc_into(c, t->d_sys_poff, 2) ||
+      !alloc_into(c, t->d_sys_noff, 2))
     rc = BH_ENOMEM;
+  up(t->d_sys_poff, p32, 8);
+  up(t->d_sys_noff, n32, 8);
   up(dp, src_pos, (size_t)n * dim * 8);
   up(dm, src_mass, (size_t)n * 8);
   up(t->perm, pm.data(), (size_t)n * 4);
@@ -588,6 +695,15 @@ int bh_tree_clone(const bh_tree* src, bh_tree** out) {
   cp(t->bmin, src->bmin, M * D * 8);
   cp(t->bmax, src->bmax, M * D * 8);
   cp(t->nodes, src->nodes, M * sizeof(NodeD));
+  t->nsys = src->nsys;
+  t->sys_poff = src->sys_poff;
+  t->sys_noff = src->sys_noff;
+  t->sys_loff = src->sys_loff;
+  if (rc == BH_OK && (!alloc_into(c, t->d_sys_poff, (size_t)t->nsys + 1) ||
+                      !alloc_into(c, t->d_sys_noff, (size_t)t->nsys + 1)))
+    rc = BH_ENOMEM;
+  cp(t->d_sys_poff, src->d_sys_poff, ((size_t)t->nsys + 1) * 4);
+  cp(t->d_sys_noff, src->d_sys_noff, ((size_t)t->nsys + 1) * 4);
   if (rc == BH_OK) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) rc = hip_fail(e, "clone", __FILE__, __LINE__);
@@ -605,6 +721,7 @@ int bh_tree_clone(const bh_tree* src, bh_tree** out) {
 int bh_tree_should_refit_dev(bh_tree* t, const double* d_pos, int64_t n, int32_t dim,
                              int32_t* refit_ok) {
   BH_REQ(t && refit_ok, "NULL argument");
+  BH_REQ(t->nsys == 1, "the refit-or-rebuild test applies to a single tree, not a forest");
   if (n != t->n || dim != t->dim) {
     *refit_ok = 0;
     return BH_OK;
@@ -683,6 +800,7 @@ int bh_accel_targets_dev(bh_tree* t, const double* d_tgt, int64_t m, double thet
                          double G, double* d_out) {
   BH_REQ(t && (m == 0 || (d_tgt && d_out)), "NULL argument");
   BH_REQ(m >= 0 && m < 0x7fffffffll, "bad target count");
+  BH_REQ(t->nsys == 1, "external targets need a single tree, not a forest");
   DeviceGuard g(t->ctx->device);
   return walk_mode(t->ctx, t, WALK_ACC, d_tgt, m, theta, eps, G, d_out, nullptr, nullptr, nullptr);
 }
@@ -708,6 +826,7 @@ int bh_interaction_lists(bh_tree* t, const double* tgt, int64_t m, double theta,
                          int32_t* ids, int64_t cap, int64_t* total) {
   BH_REQ(t && ptr, "NULL argument");
   BH_REQ(tgt || m == t->n, "self mode needs m == n");
+  BH_REQ(!tgt || t->nsys == 1, "external targets need a single tree, not a forest");
   bh_ctx* c = t->ctx;
   DeviceGuard g(c->device);
   Scratch s(c);
@@ -742,6 +861,7 @@ int bh_interaction_lists(bh_tree* t, const double* tgt, int64_t m, double theta,
 int bh_counters(bh_tree* t, const double* tgt, int64_t m, double theta, int64_t* out) {
   BH_REQ(t && out, "NULL argument");
   BH_REQ(tgt || m == t->n, "self mode needs m == n");
+  BH_REQ(!tgt || t->nsys == 1, "external targets need a single tree, not a forest");
   bh_ctx* c = t->ctx;
   DeviceGuard g(c->device);
   Scratch s(c);
@@ -786,6 +906,7 @@ int bh_accel_f32_targets_dev(bh_tree* t, const double* d_tgt, int64_t m, double 
                              double G, const bh_f32_opts* o, double* d_out, int64_t* stats) {
   BH_REQ(t && (m == 0 || (d_tgt && d_out)), "NULL argument");
   BH_REQ(m >= 0 && m < 0x7fffffffll, "bad target count");
+  BH_REQ(t->nsys == 1, "external targets need a single tree, not a forest");
   DeviceGuard g(t->ctx->device);
   return walk_f32(t->ctx, t, d_tgt, m, theta, eps, G, o, d_out, stats);
 }

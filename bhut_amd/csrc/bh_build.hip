@@ -30,16 +30,34 @@ __device__ __forceinline__ int64_t tb_leaves(const TopoTables* tb, int d, int64_
   return tb->leaves[d][c - tb->q[d]];
 }
 
-__global__ __launch_bounds__(256) void k_topology(int64_t N, int64_t L, int D, int64_t nslots,
-                                                  const TopoTables* __restrict__ tb,
+// One tree of a forest: its particle count and where its slots, nodes,
+// particles and leaves start in the flat arrays.  A single tree is the
+// one-system forest with zero offsets.
+struct SysInfo {
+  int64_t N, slot_off, noff, poff, loff;
+};
+
+__global__ __launch_bounds__(256) void k_topology(int64_t nsys, int64_t L, int64_t nslots,
+                                                  const SysInfo* __restrict__ sys,
+                                                  const TopoTables* __restrict__ tbs,
                                                   int32_t* __restrict__ start,
                                                   int32_t* __restrict__ count,
                                                   int32_t* __restrict__ first_child,
                                                   int32_t* __restrict__ parent,
                                                   int32_t* __restrict__ skip,
                                                   int32_t* __restrict__ leaf_order) {
-  for (int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; slot < nslots;
-       slot += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t gslot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gslot < nslots;
+       gslot += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = nsys - 1;  // last system whose slots start at or before gslot
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) / 2;
+      if (sys[mid].slot_off <= gslot) lo = mid;
+      else hi = mid - 1;
+    }
+    const SysInfo S = sys[lo];
+    const TopoTables* tb = tbs + lo;
+    const int64_t slot = gslot - S.slot_off;
+    const int64_t N = S.N;
     const int k = 63 - __clzll((unsigned long long)(slot + 1));  // depth
     const int64_t j = slot + 1 - (1ll << k);                     // position at depth k
     int64_t c = N, s = 0, r = 0, lrank = 0;
@@ -70,18 +88,18 @@ __global__ __launch_bounds__(256) void k_topology(int64_t N, int64_t L, int D, i
       idx = nidx;
     }
     if (!ok) continue;
-    start[idx] = (int32_t)s;
-    count[idx] = (int32_t)c;
-    parent[idx] = (int32_t)par;
-    skip[idx] = (int32_t)sk;
+    const int64_t gi = idx + S.noff;
+    start[gi] = (int32_t)(s + S.poff);
+    count[gi] = (int32_t)c;
+    parent[gi] = par >= 0 ? (int32_t)(par + S.noff) : -1;
+    skip[gi] = sk >= 0 ? (int32_t)(sk + S.noff) : -1;
     if (c > L) {
-      first_child[idx] = (int32_t)(1 + 2 * r);
+      first_child[gi] = (int32_t)(1 + 2 * r + S.noff);
     } else {
-      first_child[idx] = -1;
-      leaf_order[lrank] = (int32_t)idx;
+      first_child[gi] = -1;
+      leaf_order[S.loff + lrank] = (int32_t)gi;
     }
   }
-  (void)D;
 }
 
 static int topo_tables(int64_t N, int64_t L, TopoTables& tb, int& D) {
@@ -132,22 +150,60 @@ int topology_plan(int64_t N, int64_t L, int64_t* M, int64_t* nleaves, int* depth
   return BH_OK;
 }
 
+// Plan of a forest whose systems hold counts[0..nsys): per-system node and
+// leaf offsets (host) and totals.  A single tree is nsys = 1.
+int forest_plan(const int64_t* counts, int64_t nsys, int64_t L, std::vector<int64_t>& noff,
+                std::vector<int64_t>& loff, int64_t* M, int64_t* nleaves, int* depth) {
+  noff.assign(nsys + 1, 0);
+  loff.assign(nsys + 1, 0);
+  int dmax = 0;
+  for (int64_t s = 0; s < nsys; s++) {
+    int64_t Ms, ls;
+    int D;
+    int rc = topology_plan(counts[s], L, &Ms, &ls, &D);
+    if (rc) return rc;
+    noff[s + 1] = noff[s] + Ms;
+    loff[s + 1] = loff[s] + ls;
+    dmax = std::max(dmax, D);
+  }
+  if (noff[nsys] > 0x7fffffffll) {
+    set_error("forest of %lld nodes exceeds the int32 index range", (long long)noff[nsys]);
+    return BH_EOVERFLOW;
+  }
+  *M = noff[nsys];
+  *nleaves = loff[nsys];
+  *depth = dmax;
+  return BH_OK;
+}
+
 int topology(bh_ctx* c, bh_tree* t) {
-  const int64_t N = t->n, L = t->leaf;
-  TopoTables tb;
-  int D = 0;
-  int rc = topo_tables(N, L, tb, D);
-  if (rc) return rc;
-  Scratch s(c);
-  TopoTables* d_tb = s.get<TopoTables>(1);
-  if (!d_tb) return BH_ENOMEM;
-  BH_HIP(hipMemcpyAsync(d_tb, &tb, sizeof(tb), hipMemcpyHostToDevice, c->stream));
-  const int64_t nslots = (2ll << D) - 1;
-  const int nb = (int)std::min<int64_t>(cdiv(nslots, 256), 65536);
-  hipLaunchKernelGGL(k_topology, dim3(nb), dim3(256), 0, c->stream, N, L, D, nslots, d_tb, t->start,
+  const int64_t L = t->leaf, nsys = t->nsys;
+  std::vector<TopoTables> tbs(nsys);
+  std::vector<SysInfo> info(nsys);
+  int64_t slots = 0;
+  for (int64_t s = 0; s < nsys; s++) {
+    const int64_t N = t->sys_poff[s + 1] - t->sys_poff[s];
+    int D = 0;
+    int rc = topo_tables(N, L, tbs[s], D);
+    if (rc) return rc;
+    info[s].N = N;
+    info[s].slot_off = slots;
+    info[s].noff = t->sys_noff[s];
+    info[s].poff = t->sys_poff[s];
+    info[s].loff = t->sys_loff[s];
+    slots += (2ll << D) - 1;
+  }
+  Scratch sc(c);
+  TopoTables* d_tb = sc.get<TopoTables>(nsys);
+  SysInfo* d_info = sc.get<SysInfo>(nsys);
+  if (!d_tb || !d_info) return BH_ENOMEM;
+  BH_HIP(hipMemcpyAsync(d_tb, tbs.data(), nsys * sizeof(TopoTables), hipMemcpyHostToDevice, c->stream));
+  BH_HIP(hipMemcpyAsync(d_info, info.data(), nsys * sizeof(SysInfo), hipMemcpyHostToDevice, c->stream));
+  const int nb = (int)std::min<int64_t>(cdiv(slots, 256), 65536);
+  hipLaunchKernelGGL(k_topology, dim3(nb), dim3(256), 0, c->stream, nsys, L, slots, d_info, d_tb, t->start,
                      t->count, t->first_child, t->parent, t->skip, t->leaf_order);
   BH_LAUNCH_CHECK();
-  // the host copy of tb must outlive the async copy
+  // the host tables must outlive the asynchronous copies
   BH_HIP(hipStreamSynchronize(c->stream));
   return BH_OK;
 }

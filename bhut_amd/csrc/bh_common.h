@@ -105,6 +105,13 @@ struct bh_tree {
   int depth = 0;
   bool imported = false;  // built from host TreeData arrays
   bool binary = true;     // every internal node has two children (refit needs it)
+  // forest of independent systems (nsys = 1 for a single tree): system s
+  // owns sorted particles [sys_poff[s], sys_poff[s+1]), nodes from
+  // sys_noff[s] (its root) and leaves from sys_loff[s] in leaf_order
+  int64_t nsys = 1;
+  std::vector<int64_t> sys_poff{0, 0}, sys_noff{0, 0}, sys_loff{0, 0};
+  int32_t* d_sys_poff = nullptr;  // nsys + 1
+  int32_t* d_sys_noff = nullptr;  // nsys + 1
   // particles in Morton order
   double4* spart = nullptr;   // n: x, y, z (0 in 2-D), m
   int32_t* perm = nullptr;    // n: sorted index -> original index
@@ -164,6 +171,12 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // ------------------------------------------------------------ host entry points per TU
 // bh_sort.hip
 int bbox(bh_ctx* c, const double* pos, int64_t n, int dim, double* d_box);
+// per-system boxes (nsys x 2*dim) and Morton codes relative to each box
+int bbox_seg(bh_ctx* c, const double* pos, const int32_t* d_poff, int64_t nsys, int dim, double* d_boxes);
+int morton_seg(bh_ctx* c, const double* pos, int64_t n, int dim, const int32_t* d_poff, int64_t nsys,
+               const double* d_boxes, uint64_t* keys, uint32_t* vals);
+// keys[i] = system of original particle vals[i]
+int system_keys(bh_ctx* c, const uint32_t* vals, int64_t n, const int32_t* d_poff, int64_t nsys, uint64_t* keys);
 int morton_iota(bh_ctx* c, const double* pos, int64_t n, int dim, const double* d_box, int clamp,
                 uint64_t* keys, uint32_t* vals);
 int radix_sort(bh_ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int bits, uint64_t** keys_out,
@@ -173,6 +186,8 @@ int gather(bh_ctx* c, const double* pos, const double* mass, const uint32_t* per
            int dim, double4* spart);
 // bh_build.hip
 int topology_plan(int64_t N, int64_t L, int64_t* M, int64_t* nleaves, int* depth);
+int forest_plan(const int64_t* counts, int64_t nsys, int64_t L, std::vector<int64_t>& noff,
+                std::vector<int64_t>& loff, int64_t* M, int64_t* nleaves, int* depth);
 int topology(bh_ctx* c, bh_tree* t);
 int upward(bh_ctx* c, bh_tree* t, int refit);
 int pack_nodes(bh_ctx* c, bh_tree* t);
@@ -195,6 +210,18 @@ int direct(bh_ctx* c, const double* src, const double* mass, int64_t n, const do
 // numpy.minimum / numpy.maximum: a NaN in either operand wins.
 __device__ __forceinline__ double np_min(double a, double b) { return (a < b || a != a) ? a : b; }
 __device__ __forceinline__ double np_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
+// System owning particle i of a forest: the last s with poff[s] <= i.
+// Systems are consecutive runs both in input order and in sorted order.
+__device__ __forceinline__ int64_t find_sys(const int32_t* __restrict__ poff, int64_t nsys, int64_t i) {
+  int64_t lo = 0, hi = nsys - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) / 2;
+    if (poff[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int lane = __lane_id();

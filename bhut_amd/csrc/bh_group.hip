@@ -157,11 +157,24 @@ int f32_prepare(bh_ctx* c, bh_tree* t) {
 }
 
 // ---------------------------------------------------------------- groups
+// Target range of group g: runs of gs targets, or (forest) the table gt0
+// whose groups never straddle two systems.
+__device__ __forceinline__ void group_range(int64_t g, int64_t m, int gs, const int32_t* __restrict__ gt0,
+                                            int64_t& t0, int64_t& t1) {
+  if (gt0) {
+    t0 = gt0[g];
+    t1 = gt0[g + 1];
+  } else {
+    t0 = g * gs;
+    t1 = t0 + gs < m ? t0 + gs : m;
+  }
+}
+
 __global__ __launch_bounds__(64) void k_group_box(const double4* __restrict__ tpos, int64_t m, int gs,
-                                                  GroupBox* __restrict__ boxes) {
+                                                  const int32_t* __restrict__ gt0, GroupBox* __restrict__ boxes) {
   const int64_t g = blockIdx.x;
-  const int64_t t0 = g * gs;
-  const int64_t t1 = t0 + gs < m ? t0 + gs : m;
+  int64_t t0, t1;
+  group_range(g, m, gs, gt0, t0, t1);
   double lo[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
   double hi[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
   for (int64_t i = t0 + threadIdx.x; i < t1; i += 64) {
@@ -213,12 +226,12 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
                                                    const int64_t* __restrict__ coff,
                                                    const int64_t* __restrict__ koff,
                                                    CellEnt* __restrict__ clist, ChunkEnt* __restrict__ klist,
-                                                   uint32_t* __restrict__ err) {
+                                                   uint32_t* __restrict__ err, const int32_t* __restrict__ groot) {
   extern __shared__ int32_t stack[];
   const int64_t g = blockIdx.x;
   const int lane = threadIdx.x;
   const GroupBox B = boxes[g];
-  if (lane == 0) stack[0] = 0;
+  if (lane == 0) stack[0] = groot ? groot[g] : 0;  // the group's own system
   __syncthreads();
   int sp = 1;
   int64_t nc = 0, nk = 0, np = 0;
@@ -422,7 +435,8 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     const uint32_t* __restrict__ torig, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
     const double* __restrict__ center, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
-    const ChunkEnt* __restrict__ klist, float eps2, double G, double* __restrict__ out) {
+    const ChunkEnt* __restrict__ klist, float eps2, double G, double* __restrict__ out,
+    const int32_t* __restrict__ gt0) {
   __shared__ float4 tiles[EV_WAVES][3][64];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const int64_t slot = (int64_t)blockIdx.x * EV_WAVES + w;
@@ -430,8 +444,8 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
   const int64_t g = order ? (int64_t)order[slot] : slot;
   float4 (*tile)[64] = tiles[w];
   const int lane = threadIdx.x % 64;
-  const int64_t t0 = g * gs;
-  const int64_t t1 = t0 + gs < m ? t0 + gs : m;
+  int64_t t0, t1;
+  group_range(g, m, gs, gt0, t0, t1);
   const GroupBox B = boxes[g];
   float tx[TPL], ty[TPL], tz[TPL];
   int64_t ti[TPL];
@@ -572,11 +586,11 @@ template <int TPL, int CH>
 static void launch_eval(bh_ctx* c, int64_t ng, const int32_t* order, bool zeps, bool self, bool kahan, bool quad,
                         int64_t m, int gs, const double4* tpos, const uint32_t* torig, const bh_tree* t,
                         const GroupBox* boxes, const int64_t* coff, const CellEnt* clist, const int64_t* koff,
-                        const ChunkEnt* klist, float eps2, double G, double* out) {
+                        const ChunkEnt* klist, float eps2, double G, double* out, const int32_t* gt0) {
 #define BH_EV(Z, S, K, Q)                                                                                       \
   hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3((unsigned)cdiv(ng, EV_WAVES)), dim3(64 * EV_WAVES), 0, \
                      c->stream, ng, order, m, gs, tpos, torig, t->spart32, t->pleaf, t->center, t->dim, boxes, coff,  \
-                     clist, koff, klist, eps2, G, out)
+                     clist, koff, klist, eps2, G, out, gt0)
 #define BH_EV_Q(Z, S, K) \
   if (quad) BH_EV(Z, S, K, true); else BH_EV(Z, S, K, false)
 #define BH_EV_K(Z, S) \
@@ -637,7 +651,26 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     tpos = t->spart;
     torig = reinterpret_cast<const uint32_t*>(t->perm);
   }
-  const int64_t ng = cdiv(m, gs);
+  // groups: runs of gs targets; a forest restarts the runs at every system
+  // and walks each group from its own system's root
+  int64_t ng = cdiv(m, gs);
+  std::vector<int32_t> hg;
+  const int32_t* gt0 = nullptr;
+  const int32_t* groot = nullptr;
+  if (!tgt && t->nsys > 1) {
+    hg.reserve(2 * (ng + t->nsys) + 1);
+    for (int64_t sy = 0; sy < t->nsys; sy++)
+      for (int64_t a = t->sys_poff[sy]; a < t->sys_poff[sy + 1]; a += gs) hg.push_back((int32_t)a);
+    ng = (int64_t)hg.size();
+    hg.push_back((int32_t)m);
+    for (int64_t sy = 0, g = 0; sy < t->nsys; sy++)
+      for (int64_t a = t->sys_poff[sy]; a < t->sys_poff[sy + 1]; a += gs, g++) hg.push_back((int32_t)t->sys_noff[sy]);
+    int32_t* dg = s.get<int32_t>(hg.size());
+    if (!dg) return BH_ENOMEM;
+    BH_HIP(hipMemcpyAsync(dg, hg.data(), hg.size() * 4, hipMemcpyHostToDevice, c->stream));
+    gt0 = dg;
+    groot = dg + ng + 1;
+  }
   GroupBox* boxes = s.get<GroupBox>(ng);
   int64_t* cnt = s.get<int64_t>(3 * ng);
   int64_t* offs = s.get<int64_t>(2 * (ng + 1));
@@ -645,7 +678,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   if (!boxes || !cnt || !offs || !err) return BH_ENOMEM;
   int rc = fill_u32(c, err, 0u, 1);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_group_box, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, boxes);
+  hipLaunchKernelGGL(k_group_box, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, gt0, boxes);
   const double theta2 = theta * theta;
   const size_t lds = (size_t)stack_cap * sizeof(int32_t);
   {
@@ -653,11 +686,11 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     if (t->dim == 3)
       hipLaunchKernelGGL((k_group_walk<3, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
                          t->quad, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err);
+                         nullptr, nullptr, nullptr, err, groot);
     else
       hipLaunchKernelGGL((k_group_walk<2, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
                          t->quad, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err);
+                         nullptr, nullptr, nullptr, err, groot);
     BH_LAUNCH_CHECK();
   }
   std::vector<int64_t> hc(3 * ng);
@@ -676,7 +709,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   for (int64_t g = 0; g < ng; g++) {
     ho[g + 1] = ho[g] + hc[g];
     ho[ng + 1 + g + 1] = ho[ng + 1 + g] + hc[ng + g];
-    const int64_t tg = std::min<int64_t>(gs, m - g * gs);
+    const int64_t tg = gt0 ? hg[g + 1] - hg[g] : std::min<int64_t>(gs, m - g * gs);
     cell_int += hc[g] * tg;
     pair_int += hc[2 * ng + g] * tg;
   }
@@ -691,11 +724,11 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     if (t->dim == 3)
       hipLaunchKernelGGL((k_group_walk<3, true>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
                          t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr, offs,
-                         offs + ng + 1, clist, klist, err);
+                         offs + ng + 1, clist, klist, err, groot);
     else
       hipLaunchKernelGGL((k_group_walk<2, true>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
                          t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr, offs,
-                         offs + ng + 1, clist, klist, err);
+                         offs + ng + 1, clist, klist, err, groot);
     BH_LAUNCH_CHECK();
   }
   const float eps2 = (float)(eps * eps);
@@ -722,13 +755,13 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     const int64_t* coff = offs;
     const int64_t* koff = offs + ng + 1;
     if (gs == 64) {
-      if (ch == 16) launch_eval<1, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
-      else if (ch == 32) launch_eval<1, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
-      else launch_eval<1, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      if (ch == 16) launch_eval<1, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
+      else if (ch == 32) launch_eval<1, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
+      else launch_eval<1, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
     } else {
-      if (ch == 16) launch_eval<2, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
-      else if (ch == 32) launch_eval<2, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
-      else launch_eval<2, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out);
+      if (ch == 16) launch_eval<2, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
+      else if (ch == 32) launch_eval<2, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
+      else launch_eval<2, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
     }
     BH_LAUNCH_CHECK();
   }

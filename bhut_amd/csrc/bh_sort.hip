@@ -139,12 +139,8 @@ __device__ __forceinline__ uint64_t spread2(uint64_t x) {
 // external targets, which may lie outside the box: they are clamped into it
 // (their codes only order the targets for coherent waves).
 template <int DIM>
-__global__ __launch_bounds__(256) void k_morton(const double* __restrict__ pos, int64_t n,
-                                                const double* __restrict__ box, int clamp,
-                                                uint64_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ uint64_t morton_code(const double* __restrict__ p, const double* __restrict__ box,
+                                                int clamp) {
   const double cap = 1.0 - 1e-12;
   uint64_t code = 0;
 #pragma unroll
@@ -153,7 +149,7 @@ __global__ __launch_bounds__(256) void k_morton(const double* __restrict__ pos, 
     const double extent = hi - lo;
     const double ext = (extent <= 0) ? 1.0 : extent;
     const double off = (extent <= 0) ? 0.5 : 0.0;
-    double v = (pos[i * DIM + d] - lo) / ext + off;
+    double v = (p[d] - lo) / ext + off;
     v = (v < cap || v != v) ? v : cap;
     if (clamp) v = (v > 0.0) ? v : 0.0;  // also maps NaN to 0
     uint64_t q;
@@ -166,7 +162,117 @@ __global__ __launch_bounds__(256) void k_morton(const double* __restrict__ pos, 
     }
     code |= (DIM == 3 ? spread3(q) : spread2(q)) << d;
   }
-  keys[i] = code;
+  return code;
+}
+
+// Morton code of particle i relative to its own system's box.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_morton_seg(const double* __restrict__ pos, int64_t n,
+                                                    const int32_t* __restrict__ poff, int64_t nsys,
+                                                    const double* __restrict__ boxes, uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = find_sys(poff, nsys, i);
+  keys[i] = morton_code<DIM>(pos + i * DIM, boxes + s * 2 * DIM, 0);
+  vals[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_system_keys(const uint32_t* __restrict__ vals, int64_t n,
+                                                     const int32_t* __restrict__ poff, int64_t nsys,
+                                                     uint64_t* __restrict__ keys) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = (uint64_t)find_sys(poff, nsys, vals[i]);
+}
+
+// One workgroup per system: NaN-propagating box of its particle run.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_bbox_seg(const double* __restrict__ pos, const int32_t* __restrict__ poff,
+                                                  double* __restrict__ boxes) {
+  const int64_t s = blockIdx.x;
+  const int64_t i0 = poff[s], i1 = poff[s + 1];
+  double lo[DIM], hi[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    lo[d] = __builtin_inf();
+    hi[d] = -__builtin_inf();
+  }
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      const double v = pos[i * DIM + d];
+      lo[d] = np_min(lo[d], v);
+      hi[d] = np_max(hi[d], v);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      lo[d] = np_min(lo[d], __shfl_xor(lo[d], off));
+      hi[d] = np_max(hi[d], __shfl_xor(hi[d], off));
+    }
+  }
+  __shared__ double sh[4][2 * DIM];
+  const int w = threadIdx.x / WAVE;
+  if (__lane_id() == 0) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      sh[w][d] = lo[d];
+      sh[w][DIM + d] = hi[d];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < DIM) {
+    const int d = threadIdx.x;
+    double a = sh[0][d], b = sh[0][DIM + d];
+    for (int k = 1; k < 4; k++) {
+      a = np_min(a, sh[k][d]);
+      b = np_max(b, sh[k][DIM + d]);
+    }
+    boxes[s * 2 * DIM + d] = a;
+    boxes[s * 2 * DIM + DIM + d] = b;
+  }
+}
+
+int bbox_seg(bh_ctx* c, const double* pos, const int32_t* d_poff, int64_t nsys, int dim, double* d_boxes) {
+  if (dim == 3)
+    hipLaunchKernelGGL(k_bbox_seg<3>, dim3((unsigned)nsys), dim3(256), 0, c->stream, pos, d_poff, d_boxes);
+  else
+    hipLaunchKernelGGL(k_bbox_seg<2>, dim3((unsigned)nsys), dim3(256), 0, c->stream, pos, d_poff, d_boxes);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+int morton_seg(bh_ctx* c, const double* pos, int64_t n, int dim, const int32_t* d_poff, int64_t nsys,
+               const double* d_boxes, uint64_t* keys, uint32_t* vals) {
+  if (n == 0) return BH_OK;
+  const unsigned nb = (unsigned)cdiv(n, 256);
+  if (dim == 3)
+    hipLaunchKernelGGL(k_morton_seg<3>, dim3(nb), dim3(256), 0, c->stream, pos, n, d_poff, nsys, d_boxes, keys, vals);
+  else
+    hipLaunchKernelGGL(k_morton_seg<2>, dim3(nb), dim3(256), 0, c->stream, pos, n, d_poff, nsys, d_boxes, keys, vals);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+int system_keys(bh_ctx* c, const uint32_t* vals, int64_t n, const int32_t* d_poff, int64_t nsys, uint64_t* keys) {
+  if (n == 0) return BH_OK;
+  hipLaunchKernelGGL(k_system_keys, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, c->stream, vals, n, d_poff, nsys,
+                     keys);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_morton(const double* __restrict__ pos, int64_t n,
+                                                const double* __restrict__ box, int clamp,
+                                                uint64_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = morton_code<DIM>(pos + i * DIM, box, clamp);
   vals[i] = (uint32_t)i;
 }
 

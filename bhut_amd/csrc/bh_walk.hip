@@ -25,7 +25,9 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
                                               double eps2, double G, double* __restrict__ out,
                                               const int64_t* __restrict__ lptr,
                                               int32_t* __restrict__ lids,
-                                              int64_t* __restrict__ stats) {
+                                              int64_t* __restrict__ stats,
+                                              const int32_t* __restrict__ sys_poff,
+                                              const int32_t* __restrict__ sys_noff, int64_t nsys) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= m) return;
   int64_t orig;
@@ -46,7 +48,8 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
   int64_t nl = 0, wpos = 0;
   int64_t c_pop = 0, c_test = 0, c_mono = 0, c_leaf = 0, c_pair = 0;
   if (MODE == WALK_LFILL) wpos = lptr[orig];
-  int v = 0;
+  // root of the target's own system (a forest walks each system alone)
+  int v = (SELF && nsys > 1) ? sys_noff[find_sys(sys_poff, nsys, t)] : 0;
   while (v >= 0) {
     const NodeD nd = nodes[v];
     if (MODE == WALK_STATS) c_pop++;
@@ -177,7 +180,8 @@ static void launch_walk(bh_ctx* c, int mode, const bh_tree* t, int64_t m, const 
   const dim3 grid((unsigned)cdiv(m, 256)), block(256);
 #define BH_W(MODE)                                                                             \
   hipLaunchKernelGGL((k_walk<DIM, SELF, MODE>), grid, block, 0, c->stream, t->nodes, t->spart,  \
-                     t->perm, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats)
+                     t->perm, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats,  \
+                     t->d_sys_poff, t->d_sys_noff, t->nsys)
   switch (mode) {
     case WALK_ACC: BH_W(WALK_ACC); break;
     case WALK_LCOUNT: BH_W(WALK_LCOUNT); break;

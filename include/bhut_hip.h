@@ -78,6 +78,20 @@ int bh_tree_import(bh_ctx *ctx, int64_t n, int64_t num_nodes, int32_t dim, int32
                    const int64_t *start, const int64_t *count, const int64_t *perm,
                    const double *src_pos, const double *src_mass, bh_tree **out);
 int bh_tree_clone(const bh_tree *t, bh_tree **out);
+/* Batched form: a forest of nsys independent systems whose particles are
+ * consecutive runs of counts[s] rows of pos/mass (counts is host memory).
+ * Every system gets exactly the tree build_tree would give it alone
+ * (build.py:19-107); one segmented sort on (system, Morton code), one
+ * topology launch and one upward pass serve all systems.  Self accelerations
+ * (bh_accel_self*, bh_accel_f32_self*), refit, lists and counters accept a
+ * forest; each system's particles feel only their own system. */
+int bh_forest_build(bh_ctx *ctx, const double *pos, const double *mass, const int64_t *counts, int64_t nsys,
+                    int32_t dim, int32_t leaf_size, bh_tree **out);
+int bh_forest_build_dev(bh_ctx *ctx, const double *d_pos, const double *d_mass, const int64_t *counts,
+                        int64_t nsys, int32_t dim, int32_t leaf_size, bh_tree **out); /* synchronises */
+/* Number of systems and their particle / node offsets (nsys + 1 entries
+ * each; either array may be NULL). */
+int bh_tree_systems(const bh_tree *t, int64_t *nsys, int64_t *particle_offsets, int64_t *node_offsets);
 void bh_tree_free(bh_tree *t);
 int bh_tree_shape(const bh_tree *t, int64_t *n, int64_t *num_nodes, int32_t *dim,
                   int32_t *leaf_size, int32_t *depth);
