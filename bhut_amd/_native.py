@@ -67,6 +67,12 @@ _SIGS = {
     "bh_counters": (ctypes.c_int, [vp, vp, i64, dbl, vp]),
     "bh_direct": (ctypes.c_int, [vp, vp, vp, i64, vp, i64, i32, i32, dbl, dbl, vp, vp]),
     "bh_direct_dev": (ctypes.c_int, [vp, vp, vp, i64, vp, i64, i32, i32, dbl, dbl, vp, vp]),
+    "bh_sim_create": (ctypes.c_int, [vp, vp, vp, vp, i64, i32, i32, dbl, dbl, dbl, i32, vp, ctypes.POINTER(vp)]),
+    "bh_sim_run": (ctypes.c_int, [vp, dbl, i64, i32, i32]),
+    "bh_sim_accelerations": (ctypes.c_int, [vp, vp]),
+    "bh_sim_set_list_capacity": (ctypes.c_int, [vp, i64, i64]),
+    "bh_sim_info": (ctypes.c_int, [vp, vp]),
+    "bh_sim_destroy": (None, [vp]),
 }
 
 _lib = None
@@ -82,6 +88,15 @@ def load():
                 raise RuntimeError(
                     f"{LIB_PATH} is not built; build it with `python -c 'import __graft_entry__ as g; g.build()'`"
                     " or `make -C bhut_amd/csrc`")
+            # One HIP runtime per process: PyTorch-ROCm ships its own
+            # libamdhip64 (needed as "libamdhip64.so", soname ".so.7").  When
+            # torch is loaded first, this library's ".so.7" dependency binds
+            # to that same copy; loaded the other way round the process would
+            # hold two runtimes and torch would find no GPU.
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             L = ctypes.CDLL(LIB_PATH)
             for name, (res, args) in _SIGS.items():
                 fn = getattr(L, name)
@@ -381,3 +396,44 @@ def direct(src, mass, targets=None, *, softening=0.0, G=1.0, potential=False, co
     check(load().bh_direct(c.handle, ptr(s), ptr(m), len(s), ptr(t), len(t), s.shape[1], int(self_mode),
                            float(softening), float(G), ptr(acc), ptr(pot)))
     return (acc, pot) if potential else acc
+
+
+class Simulation:
+    """Owner of a bh_sim*: KDK leapfrog over device buffers (pointers of
+    float64 (n, dim) positions / velocities and (n,) masses, which must stay
+    alive while this object lives)."""
+
+    INFO_KEYS = ("steps", "rebuilds", "refits", "graph_launches", "graph_captures", "list_regrows",
+                 "cell_capacity", "chunk_capacity")
+
+    def __init__(self, pos_ptr: int, vel_ptr: int, mass_ptr: int, n: int, dim: int, *, leaf_size: int = 32,
+                 theta: float = 0.5, softening: float = 0.0, G: float = 1.0, precision: int = 32,
+                 opts: F32Opts | None = None, context: Context | None = None):
+        self.context = context or ctx()
+        h = vp()
+        check(load().bh_sim_create(self.context.handle, vp(pos_ptr), vp(vel_ptr), vp(mass_ptr), int(n), int(dim),
+                                   int(leaf_size), float(theta), float(softening), float(G), int(precision),
+                                   ctypes.byref(opts) if opts is not None else None, ctypes.byref(h)))
+        self.handle = h
+
+    def run(self, dt: float, steps: int, rebuild_every: int = 0, use_graph: bool = True) -> None:
+        check(load().bh_sim_run(self.handle, float(dt), int(steps), int(rebuild_every), int(bool(use_graph))))
+
+    def accelerations_dev(self, out_ptr: int) -> None:
+        check(load().bh_sim_accelerations(self.handle, vp(out_ptr)))
+
+    def set_list_capacity(self, cells: int, chunks: int) -> None:
+        check(load().bh_sim_set_list_capacity(self.handle, int(cells), int(chunks)))
+
+    def info(self) -> dict:
+        out = np.zeros(8, np.int64)
+        check(load().bh_sim_info(self.handle, ptr(out)))
+        return dict(zip(self.INFO_KEYS, (int(v) for v in out)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and _lib is not None:
+                _lib.bh_sim_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

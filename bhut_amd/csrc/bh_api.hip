@@ -52,7 +52,7 @@ void* dalloc(bh_ctx* c, size_t bytes) {
   }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, rs);
-  if (e != hipSuccess) {
+  if (e != hipSuccess && !c->capturing) {
     (void)hipGetLastError();
     (void)hipStreamSynchronize(c->stream);
     trim_locked(c);
@@ -62,6 +62,10 @@ void* dalloc(bh_ctx* c, size_t bytes) {
       set_error("device allocation of %zu bytes failed", rs);
       return nullptr;
     }
+  } else if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device allocation of %zu bytes failed during graph capture", rs);
+    return nullptr;
   }
   c->live[p] = rs;
   return p;
@@ -72,6 +76,10 @@ void dfree(bh_ctx* c, void* p) {
   std::lock_guard<std::mutex> g(c->mu);
   auto it = c->live.find(p);
   if (it == c->live.end()) return;
+  if (c->retain) {  // a graph being captured still refers to it
+    c->retain->push_back(p);
+    return;
+  }
   c->free_blocks.emplace(it->second, p);
   c->cached_bytes += it->second;
   c->live.erase(it);
@@ -89,7 +97,7 @@ static hipEvent_t prof_event(bh_ctx* c) {
 }
 
 ProfScope::ProfScope(bh_ctx* ctx, const char* name) : c(ctx) {
-  if (!c->prof) return;
+  if (!c->prof || c->capturing) return;
   bh_ctx::ProfRec r{name, prof_event(c), prof_event(c)};
   (void)hipEventRecord(r.a, c->stream);
   idx = c->prof_recs.size();
@@ -120,25 +128,11 @@ struct DeviceGuard {
   }
 };
 
-#define BH_REQ(cond, ...)          \
-  do {                             \
-    if (!(cond)) {                 \
-      ::bh::set_error(__VA_ARGS__); \
-      return BH_EINVAL;            \
-    }                              \
-  } while (0)
-
-#define BH_TRY(expr)           \
-  do {                         \
-    int _rc = (expr);          \
-    if (_rc != BH_OK) return _rc; \
-  } while (0)
-
 static void free_tree_arrays(bh_tree* t) {
   bh_ctx* c = t->ctx;
   void* ps[] = {t->spart, t->perm,  t->start, t->count, t->first_child, t->parent, t->skip,
                 t->leaf_order, t->center, t->half, t->mass, t->com, t->bmin, t->bmax, t->gbox,
-                t->nodes, t->spart32, t->pleaf, t->quad, t->d_sys_poff, t->d_sys_noff};
+                t->nodes, t->spart32, t->pleaf, t->quad, t->d_sys_poff, t->d_sys_noff, t->d_gtab};
   for (void* p : ps) dfree(c, p);
 }
 
@@ -193,15 +187,9 @@ static int build_dev(bh_ctx* c, const double* pos, const double* mass, const int
   int64_t n = 0;
   for (int64_t s = 0; s < nsys; s++) n += counts[s];
   t->n = n;
-  Scratch s(c);
   if (!alloc_into(c, t->gbox, 2 * (size_t)dim * nsys) || !alloc_into(c, t->spart, (size_t)n) ||
       !alloc_into(c, t->perm, (size_t)n))
     return BH_ENOMEM;
-  uint64_t* k0 = s.get<uint64_t>(n);
-  uint64_t* k1 = s.get<uint64_t>(n);
-  uint32_t* v1 = s.get<uint32_t>(n);
-  if (!k0 || !k1 || !v1) return BH_ENOMEM;
-  uint32_t* v0 = reinterpret_cast<uint32_t*>(t->perm);
   // the (N, leaf) tables give M before the node arrays are allocated
   BH_TRY(set_systems(t, counts, nsys));
   t->binary = true;
@@ -210,6 +198,21 @@ static int build_dev(bh_ctx* c, const double* pos, const double* mass, const int
     ProfScope p(c, "build_topology");
     BH_TRY(topology(c, t));
   }
+  return rebuild_in_place(c, t, pos, mass);
+}
+
+// Everything of a build after the topology: boxes, Morton codes, the stable
+// (system, Morton) sort, the gather into sorted order and the upward pass.
+// No host synchronisation, so it may be captured into a graph.
+int rebuild_in_place(bh_ctx* c, bh_tree* t, const double* pos, const double* mass) {
+  const int64_t n = t->n, nsys = t->nsys;
+  const int dim = t->dim;
+  Scratch s(c);
+  uint64_t* k0 = s.get<uint64_t>(n);
+  uint64_t* k1 = s.get<uint64_t>(n);
+  uint32_t* v1 = s.get<uint32_t>(n);
+  if (!k0 || !k1 || !v1) return BH_ENOMEM;
+  uint32_t* v0 = reinterpret_cast<uint32_t*>(t->perm);
   {
     ProfScope p(c, "build_bbox_morton");
     if (nsys == 1) {
@@ -244,6 +247,7 @@ static int build_dev(bh_ctx* c, const double* pos, const double* mass, const int
     BH_TRY(upward(c, t, 0));
     BH_TRY(pack_nodes(c, t));
   }
+  t->f32_ready = false;
   return BH_OK;
 }
 

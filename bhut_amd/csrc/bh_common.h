@@ -35,6 +35,21 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
 #define BH_LAUNCH_CHECK() BH_HIP(hipGetLastError())
 
+// argument check: records the message and returns BH_EINVAL
+#define BH_REQ(cond, ...)            \
+  do {                               \
+    if (!(cond)) {                   \
+      ::bh::set_error(__VA_ARGS__);  \
+      return BH_EINVAL;              \
+    }                                \
+  } while (0)
+
+#define BH_TRY(expr)                  \
+  do {                                \
+    int _rc = (expr);                 \
+    if (_rc != BH_OK) return _rc;     \
+  } while (0)
+
 // ------------------------------------------------------------ records
 // Traversal record of one node (64 B, one cache line).  `skip` is the next
 // node of bhut's pop order once this node's subtree is done; the LIFO stack
@@ -73,8 +88,11 @@ struct bh_ctx {
   // small pinned staging buffer for flags / counters read back by the host
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
-  // FP32-form list capacities remembered between calls (entries per call)
-  int64_t f32_mono_cap = 0, f32_leaf_cap = 0;
+  // While a stream capture records work into a graph, blocks released by
+  // scoped scratch are kept on this list instead (the graph still uses them)
+  // and no host synchronisation or event profiling may happen.
+  std::vector<void*>* retain = nullptr;
+  bool capturing = false;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   struct ProfRec {
@@ -138,6 +156,12 @@ struct bh_tree {
   double* quad = nullptr;  // M*6: xx, yy, zz, xy, xz, yz
   bool f32_ready = false;
   bool quad_ready = false;
+  // forest group table of the FP32 form for group size gtab_gs: ng + 1
+  // first targets, then ng roots (host copy kept for statistics)
+  int32_t* d_gtab = nullptr;
+  int gtab_gs = 0;
+  int64_t gtab_ng = 0;
+  std::vector<int32_t> h_gtab;
 };
 
 namespace bh {
@@ -200,8 +224,21 @@ int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t 
               double eps, double G, double* out, const int64_t* lptr, int32_t* lids, int64_t* stats);
 // bh_group.hip
 int f32_prepare(bh_ctx* c, bh_tree* t);
+// Capture-safe variant of the FP32 self walk: list offsets are scanned on the
+// device into lists of fixed capacity (cap_cells, cap_chunks); no host
+// synchronisation.  When a list would exceed its capacity, bit 1 of *err is
+// set, the lists and the evaluation are skipped, and need[0..1] receive the
+// required capacities.  *err must be zero on entry.
+struct F32Plan {
+  int64_t cap_cells = 0, cap_chunks = 0;
+  uint32_t* err = nullptr;  // device
+  int64_t* need = nullptr;  // device, 2 entries
+};
 int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, double eps,
-             double G, const bh_f32_opts* o, double* out, int64_t* stats);
+             double G, const bh_f32_opts* o, double* out, int64_t* stats, const F32Plan* plan = nullptr);
+// bh_api.hip: a new Morton order, particle order and monopoles for a tree
+// whose topology (a function of the system sizes) is kept
+int rebuild_in_place(bh_ctx* c, bh_tree* t, const double* pos, const double* mass);
 // bh_direct.hip
 int direct(bh_ctx* c, const double* src, const double* mass, int64_t n, const double* tgt,
            int64_t m, int dim, int self_mode, double eps, double G, double* acc, double* pot);

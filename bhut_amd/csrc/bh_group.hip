@@ -13,9 +13,10 @@
 //     (bh.py:586-601).  Lanes are compacted with ballots into the cell list,
 //     the leaf-chunk list and the stack.  Run twice: count, then fill.
 //  3. k_group_eval: one wavefront per group, one (or two) targets per lane.
-//     Tiles of 64 list entries are loaded cooperatively (coalesced) into LDS
-//     and read back as wave-wide broadcasts, double-buffered so the next
-//     tile's HBM reads overlap this tile's arithmetic.  All interactions are
+//     Tiles of 64 list entries are loaded cooperatively (coalesced) into one
+//     LDS tile per wavefront and read back as wave-wide broadcasts; the next
+//     tile's HBM reads are in flight in registers while the current tile is
+//     consumed, and groups run longest first.  All interactions are
 //     FP32 on coordinates relative to the group centre: sources are stored
 //     relative to their leaf centre and shifted by the FP64-formed
 //     leaf-to-group offset, so precision follows the leaf size, not the
@@ -228,6 +229,8 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
                                                    CellEnt* __restrict__ clist, ChunkEnt* __restrict__ klist,
                                                    uint32_t* __restrict__ err, const int32_t* __restrict__ groot) {
   extern __shared__ int32_t stack[];
+  // lists of fixed capacity that would overflow are not written at all
+  if (FILL && (__builtin_amdgcn_readfirstlane(*err) & 2u)) return;
   const int64_t g = blockIdx.x;
   const int lane = threadIdx.x;
   const GroupBox B = boxes[g];
@@ -436,11 +439,12 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     const double* __restrict__ center, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
     const ChunkEnt* __restrict__ klist, float eps2, double G, double* __restrict__ out,
-    const int32_t* __restrict__ gt0) {
+    const int32_t* __restrict__ gt0, const uint32_t* __restrict__ err) {
   __shared__ float4 tiles[EV_WAVES][3][64];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const int64_t slot = (int64_t)blockIdx.x * EV_WAVES + w;
   if (slot >= ngroups) return;  // whole wavefront
+  if (__builtin_amdgcn_readfirstlane(*err) != 0u) return;  // lists incomplete
   const int64_t g = order ? (int64_t)order[slot] : slot;
   float4 (*tile)[64] = tiles[w];
   const int lane = threadIdx.x % 64;
@@ -581,16 +585,87 @@ __global__ void k_group_cost_keys(int64_t ng, const int64_t* __restrict__ ccount
   idx[g] = (uint32_t)g;
 }
 
+// Exclusive scans of the per-group cell and chunk counts into list offsets,
+// on the device (one workgroup), with the capacity check of the fixed-size
+// lists: totals above capacity set bit 1 of *err.
+__global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* __restrict__ cnt,
+                                                     int64_t* __restrict__ offs, int64_t cap_c, int64_t cap_k,
+                                                     uint32_t* __restrict__ err, int64_t* __restrict__ need) {
+  constexpr int IT = 8;
+  __shared__ int64_t wsum[2][16];
+  const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+  int64_t* coff = offs;
+  int64_t* koff = offs + ng + 1;
+  int64_t carry_c = 0, carry_k = 0;
+  for (int64_t base = 0; base < ng; base += 1024 * IT) {
+    int64_t vc[IT], vk[IT];
+    int64_t sc = 0, sk = 0;
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+      const int64_t g = base + (int64_t)tid * IT + i;
+      vc[i] = g < ng ? cnt[g] : 0;
+      vk[i] = g < ng ? cnt[ng + g] : 0;
+      sc += vc[i];
+      sk += vk[i];
+    }
+    int64_t ic = sc, ik = sk;  // inclusive wave scan of the thread sums
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t yc = __shfl_up(ic, off), yk = __shfl_up(ik, off);
+      if (lane >= off) {
+        ic += yc;
+        ik += yk;
+      }
+    }
+    if (lane == 63) {
+      wsum[0][w] = ic;
+      wsum[1][w] = ik;
+    }
+    __syncthreads();
+    int64_t pc = carry_c + ic - sc, pk = carry_k + ik - sk;
+    int64_t tc = 0, tk = 0;
+    for (int j = 0; j < 16; j++) {
+      if (j < w) {
+        pc += wsum[0][j];
+        pk += wsum[1][j];
+      }
+      tc += wsum[0][j];
+      tk += wsum[1][j];
+    }
+    __syncthreads();  // wsum is rewritten by the next chunk
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+      const int64_t g = base + (int64_t)tid * IT + i;
+      if (g < ng) {
+        coff[g] = pc;
+        koff[g] = pk;
+      }
+      pc += vc[i];
+      pk += vk[i];
+    }
+    carry_c += tc;
+    carry_k += tk;
+  }
+  if (tid == 0) {
+    coff[ng] = carry_c;
+    koff[ng] = carry_k;
+    need[0] = carry_c;
+    need[1] = carry_k;
+    if (carry_c > cap_c || carry_k > cap_k) atomicOr(err, 2u);
+  }
+}
+
 // ---------------------------------------------------------------- host
 template <int TPL, int CH>
 static void launch_eval(bh_ctx* c, int64_t ng, const int32_t* order, bool zeps, bool self, bool kahan, bool quad,
                         int64_t m, int gs, const double4* tpos, const uint32_t* torig, const bh_tree* t,
                         const GroupBox* boxes, const int64_t* coff, const CellEnt* clist, const int64_t* koff,
-                        const ChunkEnt* klist, float eps2, double G, double* out, const int32_t* gt0) {
+                        const ChunkEnt* klist, float eps2, double G, double* out, const int32_t* gt0,
+                        const uint32_t* err) {
 #define BH_EV(Z, S, K, Q)                                                                                       \
   hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3((unsigned)cdiv(ng, EV_WAVES)), dim3(64 * EV_WAVES), 0, \
                      c->stream, ng, order, m, gs, tpos, torig, t->spart32, t->pleaf, t->center, t->dim, boxes, coff,  \
-                     clist, koff, klist, eps2, G, out, gt0)
+                     clist, koff, klist, eps2, G, out, gt0, err)
 #define BH_EV_Q(Z, S, K) \
   if (quad) BH_EV(Z, S, K, true); else BH_EV(Z, S, K, false)
 #define BH_EV_K(Z, S) \
@@ -609,7 +684,7 @@ static void launch_eval(bh_ctx* c, int64_t ng, const int32_t* order, bool zeps, 
 }
 
 int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, double eps, double G,
-             const bh_f32_opts* o, double* out, int64_t* stats) {
+             const bh_f32_opts* o, double* out, int64_t* stats, const F32Plan* plan) {
   if (m == 0) return BH_OK;
   const int gs = o && o->group_size ? o->group_size : 64;
   const bool kahan = o ? o->kahan != 0 : true;
@@ -652,32 +727,46 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     torig = reinterpret_cast<const uint32_t*>(t->perm);
   }
   // groups: runs of gs targets; a forest restarts the runs at every system
-  // and walks each group from its own system's root
+  // and walks each group from its own system's root (table kept in the tree)
   int64_t ng = cdiv(m, gs);
-  std::vector<int32_t> hg;
   const int32_t* gt0 = nullptr;
   const int32_t* groot = nullptr;
   if (!tgt && t->nsys > 1) {
-    hg.reserve(2 * (ng + t->nsys) + 1);
-    for (int64_t sy = 0; sy < t->nsys; sy++)
-      for (int64_t a = t->sys_poff[sy]; a < t->sys_poff[sy + 1]; a += gs) hg.push_back((int32_t)a);
-    ng = (int64_t)hg.size();
-    hg.push_back((int32_t)m);
-    for (int64_t sy = 0, g = 0; sy < t->nsys; sy++)
-      for (int64_t a = t->sys_poff[sy]; a < t->sys_poff[sy + 1]; a += gs, g++) hg.push_back((int32_t)t->sys_noff[sy]);
-    int32_t* dg = s.get<int32_t>(hg.size());
-    if (!dg) return BH_ENOMEM;
-    BH_HIP(hipMemcpyAsync(dg, hg.data(), hg.size() * 4, hipMemcpyHostToDevice, c->stream));
-    gt0 = dg;
-    groot = dg + ng + 1;
+    if (t->gtab_gs != gs) {
+      if (c->capturing) {
+        set_error("the forest group table must exist before graph capture");
+        return BH_EINVAL;
+      }
+      std::vector<int32_t> hg;
+      for (int64_t sy = 0; sy < t->nsys; sy++)
+        for (int64_t a = t->sys_poff[sy]; a < t->sys_poff[sy + 1]; a += gs) hg.push_back((int32_t)a);
+      const int64_t n2 = (int64_t)hg.size();
+      hg.push_back((int32_t)m);
+      for (int64_t sy = 0; sy < t->nsys; sy++)
+        for (int64_t a = t->sys_poff[sy]; a < t->sys_poff[sy + 1]; a += gs) hg.push_back((int32_t)t->sys_noff[sy]);
+      dfree(c, t->d_gtab);
+      t->d_gtab = dalloc_t<int32_t>(c, hg.size());
+      t->gtab_gs = 0;
+      if (!t->d_gtab) return BH_ENOMEM;
+      BH_HIP(hipMemcpyAsync(t->d_gtab, hg.data(), hg.size() * 4, hipMemcpyHostToDevice, c->stream));
+      BH_HIP(hipStreamSynchronize(c->stream));  // hg is a temporary
+      t->h_gtab.swap(hg);
+      t->gtab_gs = gs;
+      t->gtab_ng = n2;
+    }
+    ng = t->gtab_ng;
+    gt0 = t->d_gtab;
+    groot = t->d_gtab + ng + 1;
   }
   GroupBox* boxes = s.get<GroupBox>(ng);
   int64_t* cnt = s.get<int64_t>(3 * ng);
   int64_t* offs = s.get<int64_t>(2 * (ng + 1));
-  uint32_t* err = s.get<uint32_t>(1);
+  uint32_t* err = plan ? plan->err : s.get<uint32_t>(1);
   if (!boxes || !cnt || !offs || !err) return BH_ENOMEM;
-  int rc = fill_u32(c, err, 0u, 1);
-  if (rc) return rc;
+  if (!plan) {
+    int rc = fill_u32(c, err, 0u, 1);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_group_box, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, gt0, boxes);
   const double theta2 = theta * theta;
   const size_t lds = (size_t)stack_cap * sizeof(int32_t);
@@ -693,31 +782,44 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
                          nullptr, nullptr, nullptr, err, groot);
     BH_LAUNCH_CHECK();
   }
-  std::vector<int64_t> hc(3 * ng);
-  uint32_t herr = 0;
-  BH_HIP(hipMemcpyAsync(hc.data(), cnt, 3 * ng * 8, hipMemcpyDeviceToHost, c->stream));
-  BH_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
-  BH_HIP(hipStreamSynchronize(c->stream));
-  if (herr) {
-    set_error("group walk stack overflow (stack_cap=%d); raise bh_f32_opts.stack_cap", stack_cap);
-    return BH_EOVERFLOW;
+  CellEnt* clist;
+  ChunkEnt* klist;
+  int64_t cell_int = 0, pair_int = 0, ntc = 0, ntk = 0;
+  if (plan) {
+    // offsets scanned on the device into lists of the planned capacity
+    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, offs, plan->cap_cells,
+                       plan->cap_chunks, err, plan->need);
+    BH_LAUNCH_CHECK();
+    clist = s.get<CellEnt>((size_t)std::max<int64_t>(plan->cap_cells, 1));
+    klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(plan->cap_chunks, 1));
+  } else {
+    std::vector<int64_t> hc(3 * ng);
+    uint32_t herr = 0;
+    BH_HIP(hipMemcpyAsync(hc.data(), cnt, 3 * ng * 8, hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(hipStreamSynchronize(c->stream));
+    if (herr) {
+      set_error("group walk stack overflow (stack_cap=%d); raise bh_f32_opts.stack_cap", stack_cap);
+      return BH_EOVERFLOW;
+    }
+    std::vector<int64_t> ho(2 * (ng + 1));
+    ho[0] = 0;
+    ho[ng + 1] = 0;
+    for (int64_t g = 0; g < ng; g++) {
+      ho[g + 1] = ho[g] + hc[g];
+      ho[ng + 1 + g + 1] = ho[ng + 1 + g] + hc[ng + g];
+      const int64_t tg = gt0 ? t->h_gtab[g + 1] - t->h_gtab[g] : std::min<int64_t>(gs, m - g * gs);
+      cell_int += hc[g] * tg;
+      pair_int += hc[2 * ng + g] * tg;
+    }
+    if (!tgt) pair_int -= m;  // each target's own pair contributes nothing
+    ntc = ho[ng];
+    ntk = ho[2 * ng + 1];
+    BH_HIP(hipMemcpyAsync(offs, ho.data(), 2 * (ng + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    BH_HIP(hipStreamSynchronize(c->stream));  // ho is a temporary
+    clist = s.get<CellEnt>((size_t)std::max<int64_t>(ntc, 1));
+    klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(ntk, 1));
   }
-  std::vector<int64_t> ho(2 * (ng + 1));
-  int64_t cell_int = 0, pair_int = 0;
-  ho[0] = 0;
-  ho[ng + 1] = 0;
-  for (int64_t g = 0; g < ng; g++) {
-    ho[g + 1] = ho[g] + hc[g];
-    ho[ng + 1 + g + 1] = ho[ng + 1 + g] + hc[ng + g];
-    const int64_t tg = gt0 ? hg[g + 1] - hg[g] : std::min<int64_t>(gs, m - g * gs);
-    cell_int += hc[g] * tg;
-    pair_int += hc[2 * ng + g] * tg;
-  }
-  if (!tgt) pair_int -= m;  // each target's own pair contributes nothing
-  const int64_t ntc = ho[ng], ntk = ho[2 * ng + 1];
-  BH_HIP(hipMemcpyAsync(offs, ho.data(), 2 * (ng + 1) * 8, hipMemcpyHostToDevice, c->stream));
-  CellEnt* clist = s.get<CellEnt>((size_t)std::max<int64_t>(ntc, 1));
-  ChunkEnt* klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(ntk, 1));
   if (!clist || !klist) return BH_ENOMEM;
   {
     ProfScope p(c, "k_group_walk_fill");
@@ -735,7 +837,8 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const bool zeps = !(eps2 > 0.f);
   const bool self = tgt == nullptr;
   const int32_t* order = nullptr;
-  if (!getenv("BHUT_EVAL_INDEX_ORDER")) {
+  {
+    // longest-first dispatch order of the groups
     ProfScope p(c, "k_group_order");
     uint64_t* keys = s.get<uint64_t>(ng);
     uint64_t* keys2 = s.get<uint64_t>(ng);
@@ -746,7 +849,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
                        cnt + 2 * ng, keys, idx);
     uint64_t* ko;
     uint32_t* vo;
-    rc = radix_sort(c, keys, idx, ng, 32, &ko, &vo, keys2, idx2);
+    int rc = radix_sort(c, keys, idx, ng, 32, &ko, &vo, keys2, idx2);
     if (rc) return rc;
     order = reinterpret_cast<const int32_t*>(vo);
   }
@@ -755,17 +858,17 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     const int64_t* coff = offs;
     const int64_t* koff = offs + ng + 1;
     if (gs == 64) {
-      if (ch == 16) launch_eval<1, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
-      else if (ch == 32) launch_eval<1, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
-      else launch_eval<1, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
+      if (ch == 16) launch_eval<1, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+      else if (ch == 32) launch_eval<1, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+      else launch_eval<1, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
     } else {
-      if (ch == 16) launch_eval<2, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
-      else if (ch == 32) launch_eval<2, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
-      else launch_eval<2, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0);
+      if (ch == 16) launch_eval<2, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+      else if (ch == 32) launch_eval<2, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+      else launch_eval<2, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
     }
     BH_LAUNCH_CHECK();
   }
-  if (stats) {
+  if (stats && !plan) {
     stats[0] = cell_int;
     stats[1] = pair_int;
     stats[2] = ng;

@@ -164,6 +164,34 @@ int bh_direct_dev(bh_ctx *ctx, const double *d_src, const double *d_mass, int64_
                   const double *d_targets, int64_t m, int32_t dim, int32_t self_mode,
                   double softening, double G, double *d_acc, double *d_pot);
 
+/* ------------------------------------------------------ leapfrog driver */
+/* Kick-drift-kick leapfrog over positions / velocities resident in device
+ * memory (original particle order, (n, dim) float64; the library updates
+ * them in place).  bhut has no integrator: its galaxy example rebuilds a
+ * Tree per right-hand-side call (examples/galaxy/galaxy_simulation.py:
+ * 277-284).  Each step refits the tree (refit_tree, refit.py:17-109), or
+ * rebuilds it every rebuild_every steps (Tree.rebuild, api.py:564-606), and
+ * evaluates forces with the plain FP64 walk (precision 64) or the FP32
+ * grouped form (32).  With use_graph, each kind of step (refit / rebuild)
+ * runs eagerly once, is then captured into a HIP graph and replayed; the
+ * simulation runs on a stream of its own and synchronises after every step.
+ * Calls on one simulation must not overlap. */
+typedef struct bh_sim bh_sim;
+int bh_sim_create(bh_ctx *ctx, double *d_pos, double *d_vel, const double *d_mass, int64_t n, int32_t dim,
+                  int32_t leaf_size, double theta, double softening, double G, int32_t precision,
+                  const bh_f32_opts *opts, bh_sim **out); /* also evaluates the initial forces */
+int bh_sim_run(bh_sim *sim, double dt, int64_t steps, int32_t rebuild_every, int32_t use_graph);
+/* current accelerations (n, dim) into device memory */
+int bh_sim_accelerations(bh_sim *sim, double *d_acc);
+/* capacities (entries) of the FP32 interaction lists used by captured
+ * steps; too small a capacity is detected, regrown, and the step's force
+ * redone eagerly.  Drops the captured graphs. */
+int bh_sim_set_list_capacity(bh_sim *sim, int64_t cells, int64_t chunks);
+/* 8 int64: steps, rebuilds, refits, graph launches, graph captures, list
+ * regrows, cell capacity, chunk capacity */
+int bh_sim_info(const bh_sim *sim, int64_t *out);
+void bh_sim_destroy(bh_sim *sim);
+
 #ifdef __cplusplus
 }
 #endif
