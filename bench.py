@@ -9,7 +9,14 @@ duration comes from the library's own event profiler over the same timed
 steps.  The CPU baseline is the C restatement in oracle/ (the reference's
 algorithm, OpenMP over targets) timed on this host on a bounded sample.
 
-Prints one JSON line (see --help for the knobs; defaults run in ~1-2 min).
+--workload selects the other measured configurations (each prints its own
+JSON line; the default run prints only the headline line):
+  plummer    1M Plummer self accelerations (+ a 250k-target external query)
+  ensemble   512 independent Plummer systems, one forest vs 512 single calls
+  diskhalo   16M exponential disk + Hernquist halo, theta in {0.3, 0.5, 0.7},
+             leaf / group size / stack depth autotuned per theta
+  leapfrog   4M Plummer, 100 KDK steps, refit every step, rebuild every 10,
+             steps captured in HIP graphs; FP32 vs FP64 forces
 """
 
 from __future__ import annotations
@@ -17,7 +24,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import platform
 import sys
 import time
 
@@ -43,13 +49,15 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--n", type=int, default=1_000_000)
+    p.add_argument("--workload", choices=["plummer", "ensemble", "diskhalo", "leapfrog"], default="plummer")
+    p.add_argument("--n", type=int, default=0, help="particles (0: the workload's own size)")
     p.add_argument("--theta", type=float, default=0.5)
     p.add_argument("--softening", type=float, default=0.01)
     p.add_argument("--leaf", type=int, default=32)
     p.add_argument("--form", choices=["fp32", "fp64"], default="fp32",
                    help="form whose throughput is `value` (the other is reported beside it)")
     p.add_argument("--accuracy-targets", type=int, default=4096)
+    p.add_argument("--query-targets", type=int, default=250_000, help="external query size (plummer; 0: skip)")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-plain", action="store_true", help="skip timing the plain form beside the FP32 form")
@@ -57,7 +65,78 @@ def parse():
     p.add_argument("--stack-cap", type=int, default=0, help="FP32 form LDS stack entries (0: autotuned)")
     p.add_argument("--multipole", type=int, default=2, help="FP32 form cells: 1 monopole, 2 quadrupole")
     p.add_argument("--no-kahan", action="store_true")
+    p.add_argument("--systems", type=int, default=512, help="ensemble: number of systems")
+    p.add_argument("--lf-steps", type=int, default=100, help="leapfrog: steps")
+    p.add_argument("--no-graph", action="store_true", help="leapfrog: eager steps instead of graphs")
     return p.parse_args()
+
+
+# ---------------------------------------------------------------- helpers
+class Env:
+    def __init__(self):
+        import torch
+
+        from bhut_amd import _native
+        self.torch = torch
+        self.nat = _native
+        self.dev = torch.device("cuda:0")
+        self.ctx = _native.ctx()
+        # one explicit (default-priority) stream for torch and the library, so
+        # the HIP events bracketing a step are ordered with its kernels (the
+        # library cannot be pointed at the legacy null stream)
+        self.stream = torch.cuda.Stream(self.dev)
+        torch.cuda.set_stream(self.stream)
+        self.ctx.set_stream(self.stream.cuda_stream)
+
+    def upload(self, a):
+        return self.torch.from_numpy(np.ascontiguousarray(a)).to(self.dev)
+
+    def sync(self):
+        self.torch.cuda.synchronize()
+
+    def timed(self, fn, steps, warmup):
+        """(ms per step from HIP events, wall ms per step, kernel profile)."""
+        torch = self.torch
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        self.ctx.profile(True)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        e0.record(self.stream)
+        for _ in range(steps):
+            fn()
+        e1.record(self.stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        prof = self.ctx.profile_summary()
+        self.ctx.profile(False)
+        return e0.elapsed_time(e1) / steps, wall * 1e3 / steps, prof
+
+
+def rel_err(got, ref):
+    nr = np.linalg.norm(ref, axis=1)
+    e = np.linalg.norm(got - ref, axis=1) / nr
+    return {"rms_rel_err": float(np.sqrt(np.mean(e ** 2))), "max_rel_err": float(e.max()), "targets": int(len(e))}
+
+
+def f32_options(args, best=None):
+    from bhut_amd._native import F32Opts
+    gs = best["group_size"] if best else args.group_size
+    sc = best["stack_cap"] if best else args.stack_cap
+    return F32Opts(gs, 0 if args.no_kahan else 1, sc, args.multipole)
+
+
+def roofline_f32(prof, steps, stats):
+    launches, kms = prof.get("k_group_eval_f32", (0, 0.0))
+    kavg = kms / max(launches, 1)
+    flops = FLOPS_PER_INTERACTION * int(stats[1]) + FLOPS_PER_QUAD_CELL * int(stats[0])
+    achieved = flops / (kavg * 1e-3) / 1e12 if kavg > 0 else None
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS if achieved else None, "traffic": None,
+            "kernel": "k_group_eval_f32", "kernel_avg_ms": kavg, "flops_per_launch": flops,
+            "interactions_per_launch": int(stats[0] + stats[1])}
 
 
 def cpu_baseline(x, m, theta, eps, leaf, budget_s):
@@ -98,78 +177,55 @@ def cpu_baseline(x, m, theta, eps, leaf, budget_s):
     }
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the target-sharded multi-GPU design is "
-              "described in DESIGN.md and not measured here", file=sys.stderr)
-        sys.exit(2)
-    import torch
+def base_line(args, value, ms, dtype, config):
+    return {"metric": METRIC, "value": value, "unit": "particle-accels/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
 
-    from bhut_amd import _native, workloads
 
-    dev = torch.device("cuda:0")
-    ctx = _native.ctx()
-    stream = torch.cuda.current_stream(dev)
-    ctx.set_stream(stream.cuda_stream)
-
-    N, theta, eps, G, leaf = args.n, args.theta, args.softening, 1.0, args.leaf
+# ---------------------------------------------------------------- 1M Plummer
+def workload_plummer(args, env):
+    from bhut_amd import workloads
+    nat, torch = env.nat, env.torch
+    N = args.n or 1_000_000
+    theta, eps, G, leaf = args.theta, args.softening, 1.0, args.leaf
     log(f"bench: generating Plummer N={N} seed=1")
     x, m = workloads.plummer(N, seed=1)
-    pos_t = torch.from_numpy(x).to(dev)
-    mass_t = torch.from_numpy(m).to(dev)
-    out_t = torch.empty((N, 3), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
+    pos_t, mass_t = env.upload(x), env.upload(m)
+    out_t = torch.empty((N, 3), dtype=torch.float64, device=env.dev)
+    env.sync()
 
     tuning = None
     if args.form == "fp32" and (args.group_size == 0 or args.stack_cap == 0):
         from bhut_amd import autotune
         log("bench: autotuning the FP32 form (group size, stack depth) on this input")
-        probe = _native.DeviceTree.build_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, leaf, context=ctx)
-        best, table = autotune.tune_f32(probe, theta, eps, G, multipole=args.multipole,
-                                        kahan=not args.no_kahan,
+        probe = nat.DeviceTree.build_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, leaf, context=env.ctx)
+        best, table = autotune.tune_f32(probe, theta, eps, G, multipole=args.multipole, kahan=not args.no_kahan,
                                         group_sizes=(args.group_size,) if args.group_size else (64, 128),
-                                        stack_caps=(args.stack_cap,) if args.stack_cap else (1024, 2048))
+                                        stack_caps=(args.stack_cap,) if args.stack_cap else (1024, 2048),
+                                        out_ptr=out_t.data_ptr())
         del probe
         tuning = {"chosen": best, "table": table}
         log(f"bench: autotune -> {best}")
-        f32_opts = _native.F32Opts(best["group_size"], 0 if args.no_kahan else 1, best["stack_cap"], args.multipole)
+        opts = f32_options(args, best)
     else:
-        f32_opts = _native.F32Opts(args.group_size, 0 if args.no_kahan else 1, args.stack_cap, args.multipole)
+        opts = f32_options(args)
 
     def step(form):
-        t = _native.DeviceTree.build_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, leaf, context=ctx)
+        t = nat.DeviceTree.build_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, leaf, context=env.ctx)
         if form == "fp64":
             t.accel_self_dev(out_t.data_ptr(), theta, eps, G)
         else:
-            t.accel_f32_self_dev(out_t.data_ptr(), theta, eps, G, f32_opts)
+            t.accel_f32_self_dev(out_t.data_ptr(), theta, eps, G, opts)
         return t
-
-    def timed(form):
-        for _ in range(args.warmup):
-            step(form)
-        torch.cuda.synchronize()
-        ctx.profile(True)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        w0 = time.perf_counter()
-        e0.record(stream)
-        for _ in range(args.steps):
-            step(form)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - w0
-        prof = ctx.profile_summary()
-        ctx.profile(False)
-        return e0.elapsed_time(e1) / args.steps, wall / args.steps, prof
 
     forms = [args.form] + ([] if args.no_plain or args.form == "fp64" else ["fp64"])
     res = {}
     for form in forms:
         log(f"bench: timing form {form}: {args.warmup} warmup + {args.steps} steps")
-        ms, wall_s, prof = timed(form)
-        res[form] = dict(ms=ms, wall_ms=wall_s * 1e3, prof=prof, acc=out_t.cpu().numpy())
-        log(f"bench: {form} {ms:.3f} ms/step (wall {wall_s * 1e3:.3f}); kernels: {prof}")
+        ms, wall, prof = env.timed(lambda: step(form), args.steps, args.warmup)
+        res[form] = dict(ms=ms, wall_ms=wall, prof=prof, acc=out_t.cpu().numpy())
+        log(f"bench: {form} {ms:.3f} ms/step (wall {wall:.3f}); kernels: {prof}")
 
     # interaction counts (exact device counters / stats), outside the timed region
     t = step("fp64")
@@ -178,90 +234,295 @@ def main():
     stats32 = None
     if "fp32" in res:
         out2 = torch.empty_like(out_t)
-        stats32 = t.accel_f32_self_dev(out2.data_ptr(), theta, eps, G, f32_opts)
-        torch.cuda.synchronize()
+        stats32 = t.accel_f32_self_dev(out2.data_ptr(), theta, eps, G, opts)
+        env.sync()
 
     # accuracy against FP64 direct summation on random targets
     rng = np.random.default_rng(7)
     sel = np.sort(rng.choice(N, min(args.accuracy_targets, N), replace=False))
-    ref = _native.direct(x, m, x[sel], softening=eps, G=G)
-    nr = np.linalg.norm(ref, axis=1)
-    acc_err = {}
-    for form in res:
-        e = np.linalg.norm(res[form]["acc"][sel] - ref, axis=1) / nr
-        acc_err[form] = (float(np.sqrt(np.mean(e ** 2))), float(e.max()))
-        log(f"bench: {form} rms rel err {acc_err[form][0]:.3e} max {acc_err[form][1]:.3e} "
-            f"({len(sel)} targets vs FP64 direct sum)")
-
-    # monopole-only variant of the FP32 form, for the accuracy table only
+    ref = nat.direct(x, m, x[sel], softening=eps, G=G)
+    acc_err = {form: rel_err(res[form]["acc"][sel], ref) for form in res}
+    for form, e in acc_err.items():
+        log(f"bench: {form} rms rel err {e['rms_rel_err']:.3e} max {e['max_rel_err']:.3e} vs FP64 direct sum")
     if "fp32" in res:
         out3 = torch.empty_like(out_t)
-        t.accel_f32_self_dev(out3.data_ptr(), theta, eps, G, _native.F32Opts(0, 1, 0, 1))
-        torch.cuda.synchronize()
-        e = np.linalg.norm(out3.cpu().numpy()[sel] - ref, axis=1) / nr
-        acc_err["fp32_monopole_only"] = (float(np.sqrt(np.mean(e ** 2))), float(e.max()))
-        e = np.linalg.norm(res["fp32"]["acc"][sel] - res["fp64"]["acc"][sel], axis=1) / \
-            np.linalg.norm(res["fp64"]["acc"][sel], axis=1) if "fp64" in res else None
-        if e is not None:
-            acc_err["fp32_vs_plain"] = (float(np.sqrt(np.mean(e ** 2))), float(e.max()))
+        t.accel_f32_self_dev(out3.data_ptr(), theta, eps, G, nat.F32Opts(0, 1, 0, 1))
+        env.sync()
+        acc_err["fp32_monopole_only"] = rel_err(out3.cpu().numpy()[sel], ref)
+        if "fp64" in res:
+            acc_err["fp32_vs_plain"] = rel_err(res["fp32"]["acc"][sel], res["fp64"]["acc"][sel])
 
     head = args.form
     ms = res[head]["ms"]
     if head == "fp64":
-        kname, peak = "k_walk_f64", PEAK_FP64_TFLOPS
-        inter = inter64
+        launches, kms = res[head]["prof"].get("k_walk_f64", (0, 0.0))
+        kavg = kms / max(launches, 1)
         flops = FLOPS_PER_INTERACTION * inter64
+        achieved = flops / (kavg * 1e-3) / 1e12 if kavg > 0 else None
+        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS if achieved else None, "traffic": None, "kernel": "k_walk_f64",
+                "kernel_avg_ms": kavg, "flops_per_launch": flops, "interactions_per_launch": inter64}
+        inter = inter64
     else:
-        kname, peak = "k_group_eval_f32", PEAK_FP32_TFLOPS
-        inter = int(stats32[0] + stats32[1])
-        flops = FLOPS_PER_INTERACTION * int(stats32[1]) + FLOPS_PER_QUAD_CELL * int(stats32[0])
-    launches, kms = res[head]["prof"].get(kname, (0, 0.0))
-    kavg = kms / max(launches, 1)
-    achieved = flops / (kavg * 1e-3) / 1e12 if kavg > 0 else None
-    # HBM bytes per launch come from the committed FETCH_SIZE/WRITE_SIZE
-    # passes (rocprofv3) of this same workload, when it matches
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "plummer1m_group_eval_traffic.json")
-    if head == "fp32" and N == 1_000_000 and leaf == 32 and f32_opts.group_size in (0, 64) and os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
-    line = {
-        "metric": METRIC,
-        "value": N / (ms * 1e-3),
-        "unit": "particle-accels/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32" if head == "fp32" else "fp64",
-        "data": "synthetic",
-        "config": {"workload": "plummer_self_accelerations", "n": N, "theta": theta, "softening": eps,
-                   "leaf_size": leaf, "G": G, "seed": 1, "step": "tree build + self accelerations",
-                   "form": head},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": (achieved / peak) if achieved else None, "traffic": traffic,
-                     "kernel": kname, "kernel_avg_ms": kavg,
-                     "flops_per_launch": flops, "interactions_per_launch": inter},
-        "accuracy": {f: {"rms_rel_err": acc_err[f][0], "max_rel_err": acc_err[f][1]} for f in acc_err},
-        "interactions_per_s": inter / (ms * 1e-3),
-        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in res[head]["prof"].items()},
-        "gpu": torch.cuda.get_device_name(0),
-        "f32_options": None if head != "fp32" else {
-            "group_size": f32_opts.group_size, "stack_cap": f32_opts.stack_cap, "kahan": f32_opts.kahan,
-            "multipole": f32_opts.multipole, "autotune": tuning},
-    }
+        roof = roofline_f32(res[head]["prof"], args.steps, stats32)
+        inter = roof["interactions_per_launch"]
+        # HBM bytes per launch come from the committed FETCH_SIZE/WRITE_SIZE
+        # passes (rocprofv3) of this same workload, when it matches
+        tpath = os.path.join(ROOT, "profiles", "plummer1m_group_eval_traffic.json")
+        if N == 1_000_000 and leaf == 32 and opts.group_size in (0, 64) and os.path.exists(tpath):
+            roof["traffic"] = json.load(open(tpath)).get("hbm_bytes_per_launch")
+    line = base_line(args, N / (ms * 1e-3), ms, "fp32" if head == "fp32" else "fp64",
+                     {"workload": "plummer_self_accelerations", "n": N, "theta": theta, "softening": eps,
+                      "leaf_size": leaf, "G": G, "seed": 1, "step": "tree build + self accelerations", "form": head})
+    line["roofline"] = roof
+    line["accuracy"] = acc_err
+    line["interactions_per_s"] = inter / (ms * 1e-3)
+    line["kernels_ms_per_step"] = {k: v[1] / args.steps for k, v in res[head]["prof"].items()}
+    line["gpu"] = torch.cuda.get_device_name(0)
+    line["f32_options"] = None if head != "fp32" else {
+        "group_size": opts.group_size, "stack_cap": opts.stack_cap, "kahan": opts.kahan,
+        "multipole": opts.multipole, "autotune": tuning}
     if head == "fp32" and "fp64" in res:
         line["plain_fp64"] = {"value": N / (res["fp64"]["ms"] * 1e-3), "ms_per_step": res["fp64"]["ms"],
                               "interactions_per_launch": inter64,
                               "kernels_ms_per_step": {k: v[1] / args.steps for k, v in res["fp64"]["prof"].items()}}
         line["speedup_fp32_vs_plain"] = res["fp64"]["ms"] / ms
+
+    if args.query_targets:
+        # external query on the resident tree (build excluded): the M targets
+        # uniform in [-5, 5]^3, both forms, accuracy against FP64 direct sum
+        M = args.query_targets
+        rng = np.random.default_rng(11)
+        q = rng.uniform(-5.0, 5.0, (M, 3))
+        q_t = env.upload(q)
+        qo = torch.empty((M, 3), dtype=torch.float64, device=env.dev)
+        env.sync()
+        qres = {}
+        for form in (["fp32"] if head == "fp32" else []) + ([] if args.no_plain and head == "fp32" else ["fp64"]):
+            if form == "fp32":
+                fn = lambda: t.accel_f32_targets_dev(q_t.data_ptr(), M, qo.data_ptr(), theta, eps, G, opts)
+            else:
+                fn = lambda: t.accel_targets_dev(q_t.data_ptr(), M, qo.data_ptr(), theta, eps, G)
+            qms, _, _ = env.timed(fn, max(1, args.steps // 2), 1)
+            qsel = np.sort(rng.choice(M, min(args.accuracy_targets, M), replace=False))
+            qref = nat.direct(x, m, q[qsel], softening=eps, G=G)
+            qres[form] = {"targets_per_s": M / (qms * 1e-3), "ms": qms,
+                          "accuracy": rel_err(qo.cpu().numpy()[qsel], qref)}
+            log(f"bench: query {form} {qms:.3f} ms for {M} targets; {qres[form]['accuracy']}")
+        line["query"] = {"targets": M, "distribution": "uniform [-5,5]^3, seed 11", "forms": qres}
+
     if not args.no_cpu:
         log(f"bench: CPU baseline (oracle port), budget {args.cpu_seconds:.0f} s")
         line["cpu_baseline"] = cpu_baseline(x, m, theta, eps, leaf, args.cpu_seconds)
     else:
         line["cpu_baseline"] = None
+    return line
+
+
+# ---------------------------------------------------------------- ensemble
+def workload_ensemble(args, env):
+    from bhut_amd import workloads
+    nat, torch = env.nat, env.torch
+    theta, eps, G, leaf = args.theta, args.softening, 1.0, args.leaf
+    log(f"bench: generating {args.systems} Plummer systems (seeds 1000+i)")
+    systems = workloads.ensemble(args.systems)
+    counts = np.array([len(s[0]) for s in systems], np.int64)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    x = np.concatenate([s[0] for s in systems])
+    m = np.concatenate([s[1] for s in systems])
+    N = len(x)
+    log(f"bench: ensemble of {len(counts)} systems, {N} particles (min {counts.min()}, max {counts.max()})")
+    pos_t, mass_t = env.upload(x), env.upload(m)
+    out_t = torch.empty((N, 3), dtype=torch.float64, device=env.dev)
+    env.sync()
+    opts = f32_options(args, {"group_size": args.group_size or 64, "stack_cap": args.stack_cap or 2048})
+
+    def forest_step(form):
+        f = nat.DeviceTree.build_forest_dev(pos_t.data_ptr(), mass_t.data_ptr(), counts, 3, leaf, context=env.ctx)
+        if form == "fp64":
+            f.accel_self_dev(out_t.data_ptr(), theta, eps, G)
+        else:
+            f.accel_f32_self_dev(out_t.data_ptr(), theta, eps, G, opts)
+        return f
+
+    def sequential(form):
+        for s in range(len(counts)):
+            a, b = int(off[s]), int(off[s + 1])
+            t = nat.DeviceTree.build_dev(pos_t[a:b].data_ptr(), mass_t[a:b].data_ptr(), b - a, 3, leaf,
+                                         context=env.ctx)
+            if form == "fp64":
+                t.accel_self_dev(out_t[a:b].data_ptr(), theta, eps, G)
+            else:
+                t.accel_f32_self_dev(out_t[a:b].data_ptr(), theta, eps, G, opts)
+
+    res = {}
+    plan = [("forest_fp32", lambda: forest_step("fp32"), args.steps, args.warmup),
+            ("sequential_fp32", lambda: sequential("fp32"), 1, 1),
+            ("forest_fp64", lambda: forest_step("fp64"), 1, 1),
+            ("sequential_fp64", lambda: sequential("fp64"), 1, 1)]
+    rng = np.random.default_rng(5)
+    pick = np.sort(rng.choice(len(counts), min(16, len(counts)), replace=False))
+    sels = [np.sort(rng.choice(int(counts[s]), min(512, int(counts[s])), replace=False)) for s in pick]
+    refs = [nat.direct(systems[s][0], systems[s][1], systems[s][0][sel], softening=eps, G=G)
+            for s, sel in zip(pick, sels)]
+    for name, fn, steps, warm in plan:
+        log(f"bench: timing {name}: {warm} warmup + {steps} steps")
+        ms, wall, prof = env.timed(fn, steps, warm)
+        acc = out_t.cpu().numpy()
+        got = np.concatenate([acc[off[s]:off[s + 1]][sel] for s, sel in zip(pick, sels)])
+        res[name] = {"ms": ms, "wall_ms": wall, "particle_accels_per_s": N / (ms * 1e-3),
+                     "accuracy": rel_err(got, np.concatenate(refs)),
+                     "kernels_ms_per_step": {k: v[1] / steps for k, v in prof.items()}}
+        log(f"bench: {name} {ms:.1f} ms/step; {res[name]['accuracy']}")
+        if name == "forest_fp32":
+            prof32 = prof
+    f = forest_step("fp32")
+    st = f.accel_f32_self_dev(out_t.data_ptr(), theta, eps, G, opts)
+    env.sync()
+    head = res["forest_fp32"]
+    line = base_line(args, head["particle_accels_per_s"], head["ms"], "fp32",
+                     {"workload": "plummer_ensemble_forest", "systems": int(len(counts)), "n_total": int(N),
+                      "n_min": int(counts.min()), "n_max": int(counts.max()), "theta": theta, "softening": eps,
+                      "leaf_size": leaf, "G": G, "seeds": "1000+i",
+                      "step": "forest build (segmented sort) + FP32 self accelerations, one launch per kernel"})
+    line["metric"] = "particle-accels/s, 512-system Plummer ensemble θ=0.5 (batched forest)"
+    line["roofline"] = roofline_f32(prof32, args.steps, st)
+    line["forms"] = res
+    line["speedup_forest_fp32_vs_sequential_plain"] = res["sequential_fp64"]["ms"] / head["ms"]
+    line["speedup_forest_fp32_vs_sequential_fp32"] = res["sequential_fp32"]["ms"] / head["ms"]
+    line["accuracy_sample"] = f"{len(pick)} random systems x up to 512 random targets vs FP64 direct sum"
+    line["cpu_baseline"] = None
+    return line
+
+
+# ---------------------------------------------------------------- disk + halo
+def workload_diskhalo(args, env):
+    from bhut_amd import autotune, workloads
+    nat, torch = env.nat, env.torch
+    eps, G = args.softening, 1.0
+    nd = (args.n or 16_000_000) // 2
+    log(f"bench: generating disk + halo, {nd} + {nd} particles")
+    x, m = workloads.disk_halo(nd, nd)
+    N = len(x)
+    pos_t, mass_t = env.upload(x), env.upload(m)
+    out_t = torch.empty((N, 3), dtype=torch.float64, device=env.dev)
+    env.sync()
+    rng = np.random.default_rng(9)
+    sel = np.sort(rng.choice(N, 10_000, replace=False))
+    log("bench: FP64 direct sum on the 10,000-target subset")
+    ref = nat.direct(x, m, x[sel], softening=eps, G=G)
+    rows = []
+    for theta in (0.3, 0.5, 0.7):
+        log(f"bench: theta={theta}: autotuning leaf / group size / stack depth")
+        best, table = autotune.tune_leaf_f32_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, out_t.data_ptr(),
+                                                 theta, eps, G, stack_caps=(2048,), multipole=args.multipole,
+                                                 context=env.ctx)
+        opts = f32_options(args, best)
+        leaf = best["leaf_size"]
+
+        def step():
+            t = nat.DeviceTree.build_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, leaf, context=env.ctx)
+            t.accel_f32_self_dev(out_t.data_ptr(), theta, eps, G, opts)
+            return t
+
+        ms, wall, prof = env.timed(step, max(1, args.steps // 3), 1)
+        t = step()
+        st = t.accel_f32_self_dev(out_t.data_ptr(), theta, eps, G, opts)
+        env.sync()
+        acc = rel_err(out_t.cpu().numpy()[sel], ref)
+        row = {"theta": theta, "chosen": best, "ms": ms, "particle_accels_per_s": N / (ms * 1e-3),
+               "interactions_per_s": float(st[0] + st[1]) / (ms * 1e-3), "accuracy": acc,
+               "roofline": roofline_f32(prof, 1, st), "autotune": table,
+               "kernels_ms_per_step": {k: v[1] / max(1, args.steps // 3) for k, v in prof.items()}}
+        log(f"bench: theta={theta} {best} {ms:.1f} ms/step; {acc}")
+        rows.append(row)
+        del t
+    head = rows[1]
+    line = base_line(args, head["particle_accels_per_s"], head["ms"], "fp32",
+                     {"workload": "disk_halo_self_accelerations", "n": int(N), "n_disk": nd, "n_halo": nd,
+                      "softening": eps, "G": G, "seed": 3, "theta": [0.3, 0.5, 0.7],
+                      "step": "tree build + FP32 self accelerations (autotuned per theta)"})
+    line["metric"] = "particle-accels/s, 16M disk+halo (value at θ=0.5)"
+    line["roofline"] = head["roofline"]
+    line["thetas"] = rows
+    line["accuracy_sample"] = "10,000 random particles vs FP64 direct sum over all 16M"
+    line["cpu_baseline"] = None
+    return line
+
+
+# ---------------------------------------------------------------- leapfrog
+def workload_leapfrog(args, env):
+    from bhut_amd import workloads
+    from bhut_amd.integrate import Leapfrog
+    torch = env.torch
+    N = args.n or 4_000_000
+    theta, eps, G, leaf = args.theta, args.softening, 1.0, args.leaf
+    dt, steps, rebuild = 1e-3, args.lf_steps, 10
+    log(f"bench: generating Plummer N={N} with Gaussian velocities sigma=0.3")
+    x, m = workloads.plummer(N, seed=1)
+    v = workloads.plummer_velocities(N, seed=2, sigma=0.3)
+    res = {}
+    e0 = None
+    finals = {}
+    for prec in ("fp32", "fp64"):
+        lf = Leapfrog(x, v, m, theta=theta, softening=eps, G=G, leaf_size=leaf, precision=prec)
+        if e0 is None:
+            log("bench: initial energy (FP64 direct potential)")
+            t0 = time.perf_counter()
+            e0 = lf.energy()
+            log(f"bench: E0 = {e0:.12e} ({time.perf_counter() - t0:.1f} s)")
+        env.sync()
+        e_a = torch.cuda.Event(enable_timing=True)
+        e_b = torch.cuda.Event(enable_timing=True)
+        done = 0
+        total_ms = 0.0
+        while done < steps:
+            k = min(10, steps - done)
+            e_a.record(env.stream)
+            lf.run(k, dt, rebuild_every=rebuild, use_graph=not args.no_graph)
+            e_b.record(env.stream)
+            torch.cuda.synchronize()
+            total_ms += e_a.elapsed_time(e_b)
+            done += k
+            log(f"bench: {prec} {done}/{steps} steps, {total_ms / done:.2f} ms/step")
+        log(f"bench: {prec} final energy")
+        e1 = lf.energy()
+        finals[prec] = lf.pos.cpu().numpy()
+        res[prec] = {"ms_per_step": total_ms / steps, "particle_steps_per_s": N * steps / (total_ms * 1e-3),
+                     "energy_final": e1, "rel_energy_drift": (e1 - e0) / abs(e0), "sim": lf.info()}
+        log(f"bench: {prec} {res[prec]}")
+        del lf
+    dev = np.linalg.norm(finals["fp32"] - finals["fp64"], axis=1)
+    disp = np.linalg.norm(finals["fp64"] - x, axis=1)
+    head = res["fp32"]
+    line = base_line(args, N / (head["ms_per_step"] * 1e-3), head["ms_per_step"], "fp32",
+                     {"workload": "plummer_leapfrog", "n": N, "steps": steps, "dt": dt, "rebuild_every": rebuild,
+                      "theta": theta, "softening": eps, "leaf_size": leaf, "G": G, "seed": 1,
+                      "velocities": "isotropic Gaussian sigma 0.3, seed 2",
+                      "step": "KDK: kick-drift, refit (or rebuild), forces, kick; HIP graph per step kind"})
+    line["steps"] = steps
+    line["warmup"] = 0
+    line["metric"] = "particle-steps/s, 4M Plummer KDK leapfrog (refit each step, rebuild every 10)"
+    line["unit"] = "particle-steps/s"
+    line["forms"] = res
+    line["energy_initial"] = e0
+    line["final_position_rms_deviation_fp32_vs_fp64"] = float(np.sqrt(np.mean(dev ** 2)))
+    line["final_position_rms_displacement"] = float(np.sqrt(np.mean(disp ** 2)))
+    line["roofline"] = None
+    line["cpu_baseline"] = None
+    return line
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the target-sharded multi-GPU design is "
+              "described in DESIGN.md and not measured here", file=sys.stderr)
+        sys.exit(2)
+    env = Env()
+    fn = {"plummer": workload_plummer, "ensemble": workload_ensemble, "diskhalo": workload_diskhalo,
+          "leapfrog": workload_leapfrog}[args.workload]
+    line = fn(args, env)
     print(json.dumps(line), flush=True)
 
 

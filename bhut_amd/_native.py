@@ -38,6 +38,7 @@ _SIGS = {
     "bh_ctx_stream": (vp, [vp]),
     "bh_ctx_set_stream": (ctypes.c_int, [vp, vp]),
     "bh_ctx_trim": (ctypes.c_int, [vp]),
+    "bh_ctx_set_list_budget": (ctypes.c_int, [vp, i64]),
     "bh_ctx_profile": (ctypes.c_int, [vp, i32]),
     "bh_ctx_profile_summary": (ctypes.c_int, [vp, ctypes.c_char_p, i64]),
     "bh_tree_build": (ctypes.c_int, [vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
@@ -135,6 +136,10 @@ class Context:
 
     def set_stream(self, stream_ptr) -> None:
         check(load().bh_ctx_set_stream(self.handle, vp(stream_ptr) if stream_ptr else None))
+
+    def set_list_budget(self, nbytes: int) -> None:
+        """Bytes of FP32 interaction lists held at once (0: default)."""
+        check(load().bh_ctx_set_list_budget(self.handle, int(nbytes)))
 
     def trim(self) -> None:
         check(load().bh_ctx_trim(self.handle))

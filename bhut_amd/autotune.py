@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _native
 
-_F32_KERNELS = ("k_group_walk_count", "k_group_walk_fill", "k_group_eval_f32", "f32_prepare")
+_F32_KERNELS = ("k_group_walk_count", "k_group_walk_fill", "k_group_order", "k_group_eval_f32", "f32_prepare")
 
 
 def _timed(ctx, fn, reps: int) -> float:
@@ -31,15 +31,21 @@ def _timed(ctx, fn, reps: int) -> float:
 
 def tune_f32(tree: "_native.DeviceTree", theta: float, softening: float, G: float = 1.0, *, targets=None,
              group_sizes: Iterable[int] = (64, 128), stack_caps: Iterable[int] = (1024, 2048), multipole: int = 2,
-             kahan: bool = True, reps: int = 2):
-    """Best (group_size, stack_cap) for this tree: returns (best, table)."""
+             kahan: bool = True, reps: int = 2, out_ptr: int | None = None):
+    """Best (group_size, stack_cap) for this tree: returns (best, table).
+    With out_ptr (device buffer of n x dim float64) self accelerations stay
+    on the device."""
     ctx = tree.context
     table = []
     for gs in group_sizes:
         for sc in stack_caps:
             opts = _native.F32Opts(int(gs), int(kahan), int(sc), int(multipole))
+            if out_ptr is not None and targets is None:
+                fn = lambda: tree.accel_f32_self_dev(out_ptr, theta, softening, G, opts)
+            else:
+                fn = lambda: tree.accel_f32(targets, theta, softening, G, opts)
             try:
-                ms = _timed(ctx, lambda: tree.accel_f32(targets, theta, softening, G, opts), reps)
+                ms = _timed(ctx, fn, reps)
             except RuntimeError as e:
                 if "stack" not in str(e):
                     raise
@@ -69,6 +75,31 @@ def tune_leaf_f32(positions, masses, theta: float, softening: float, G: float = 
         build_ms = sum(ms for name, (_, ms) in prof.items() if name.startswith("build_"))
         best, sub = tune_f32(tree, theta, softening, G, group_sizes=group_sizes, stack_caps=stack_caps,
                              multipole=multipole, reps=reps)
+        for r in sub:
+            table.append(dict(r, leaf_size=leaf, build_ms=build_ms,
+                              total_ms=None if r["ms"] is None else r["ms"] + build_ms))
+        del tree
+    ok = [r for r in table if r["total_ms"] is not None]
+    best = min(ok, key=lambda r: r["total_ms"])
+    return {"leaf_size": best["leaf_size"], "group_size": best["group_size"], "stack_cap": best["stack_cap"]}, table
+
+
+def tune_leaf_f32_dev(pos_ptr: int, mass_ptr: int, n: int, dim: int, out_ptr: int, theta: float,
+                      softening: float, G: float = 1.0, *, leaf_sizes: Iterable[int] = (16, 32, 64),
+                      group_sizes: Iterable[int] = (64, 128), stack_caps: Iterable[int] = (2048,),
+                      multipole: int = 2, reps: int = 1, context=None):
+    """tune_leaf_f32 on inputs already in device memory."""
+    ctx = context or _native.ctx()
+    table = []
+    for leaf in leaf_sizes:
+        tree = _native.DeviceTree.build_dev(pos_ptr, mass_ptr, n, dim, int(leaf), context=ctx)
+        ctx.profile(True)
+        tree = _native.DeviceTree.build_dev(pos_ptr, mass_ptr, n, dim, int(leaf), context=ctx)
+        prof = ctx.profile_summary()
+        ctx.profile(False)
+        build_ms = sum(ms for name, (_, ms) in prof.items() if name.startswith("build_"))
+        best, sub = tune_f32(tree, theta, softening, G, group_sizes=group_sizes, stack_caps=stack_caps,
+                             multipole=multipole, reps=reps, out_ptr=out_ptr)
         for r in sub:
             table.append(dict(r, leaf_size=leaf, build_ms=build_ms,
                               total_ms=None if r["ms"] is None else r["ms"] + build_ms))

@@ -317,6 +317,13 @@ int bh_ctx_synchronize(bh_ctx* c) {
 
 void* bh_ctx_stream(bh_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+int bh_ctx_set_list_budget(bh_ctx* c, int64_t bytes) {
+  BH_REQ(c, "ctx is NULL");
+  BH_REQ(bytes >= 0, "budget must be non-negative");
+  c->list_budget = (size_t)bytes;
+  return BH_OK;
+}
+
 int bh_ctx_set_stream(bh_ctx* c, void* stream) {
   BH_REQ(c, "ctx is NULL");
   DeviceGuard g(c->device);

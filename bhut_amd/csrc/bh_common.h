@@ -93,6 +93,8 @@ struct bh_ctx {
   // and no host synchronisation or event profiling may happen.
   std::vector<void*>* retain = nullptr;
   bool capturing = false;
+  // bytes of FP32 interaction lists materialised at once (0: default)
+  size_t list_budget = 0;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   struct ProfRec {

@@ -227,11 +227,12 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
                                                    const int64_t* __restrict__ coff,
                                                    const int64_t* __restrict__ koff,
                                                    CellEnt* __restrict__ clist, ChunkEnt* __restrict__ klist,
-                                                   uint32_t* __restrict__ err, const int32_t* __restrict__ groot) {
+                                                   uint32_t* __restrict__ err, const int32_t* __restrict__ groot,
+                                                   int64_t g0) {
   extern __shared__ int32_t stack[];
   // lists of fixed capacity that would overflow are not written at all
   if (FILL && (__builtin_amdgcn_readfirstlane(*err) & 2u)) return;
-  const int64_t g = blockIdx.x;
+  const int64_t g = g0 + blockIdx.x;
   const int lane = threadIdx.x;
   const GroupBox B = boxes[g];
   if (lane == 0) stack[0] = groot ? groot[g] : 0;  // the group's own system
@@ -576,13 +577,15 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
 // Longest-first schedule: key = complement of the group's estimated work
 // (particle pairs + 3 x accepted cells) so an ascending sort puts the most
 // expensive groups at the front of the dispatch order.
-__global__ void k_group_cost_keys(int64_t ng, const int64_t* __restrict__ ccount, const int64_t* __restrict__ pcount,
-                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= ng) return;
+__global__ void k_group_cost_keys(int64_t g0, int64_t nb, const int64_t* __restrict__ ccount,
+                                  const int64_t* __restrict__ pcount, uint64_t* __restrict__ keys,
+                                  uint32_t* __restrict__ idx) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int64_t g = g0 + i;
   const uint64_t cost = (uint64_t)pcount[g] + 3ull * (uint64_t)ccount[g];
-  keys[g] = 0xffffffffull - (cost < 0xffffffffull ? cost : 0xffffffffull);
-  idx[g] = (uint32_t)g;
+  keys[i] = 0xffffffffull - (cost < 0xffffffffull ? cost : 0xffffffffull);
+  idx[i] = (uint32_t)g;
 }
 
 // Exclusive scans of the per-group cell and chunk counts into list offsets,
@@ -656,6 +659,10 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
 }
 
 // ---------------------------------------------------------------- host
+// Bytes of interaction lists materialised at once by the host-scanned path
+// unless the context sets another budget.
+constexpr size_t LIST_BUDGET = size_t(12) << 30;
+
 template <int TPL, int CH>
 static void launch_eval(bh_ctx* c, int64_t ng, const int32_t* order, bool zeps, bool self, bool kahan, bool quad,
                         int64_t m, int gs, const double4* tpos, const uint32_t* torig, const bh_tree* t,
@@ -775,13 +782,21 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     if (t->dim == 3)
       hipLaunchKernelGGL((k_group_walk<3, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
                          t->quad, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err, groot);
+                         nullptr, nullptr, nullptr, err, groot, (int64_t)0);
     else
       hipLaunchKernelGGL((k_group_walk<2, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
                          t->quad, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err, groot);
+                         nullptr, nullptr, nullptr, err, groot, (int64_t)0);
     BH_LAUNCH_CHECK();
   }
+  // Lists are materialised for batches of consecutive groups whose lists
+  // fit LIST_BUDGET bytes (one batch unless the problem is very large);
+  // each batch is filled, ordered longest first and evaluated in turn.
+  struct Batch {
+    int64_t g0, g1;
+  };
+  std::vector<Batch> batches;
+  std::vector<int64_t> ho;
   CellEnt* clist;
   ChunkEnt* klist;
   int64_t cell_int = 0, pair_int = 0, ntc = 0, ntk = 0;
@@ -792,6 +807,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     BH_LAUNCH_CHECK();
     clist = s.get<CellEnt>((size_t)std::max<int64_t>(plan->cap_cells, 1));
     klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(plan->cap_chunks, 1));
+    batches.push_back({0, ng});
   } else {
     std::vector<int64_t> hc(3 * ng);
     uint32_t herr = 0;
@@ -802,71 +818,102 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       set_error("group walk stack overflow (stack_cap=%d); raise bh_f32_opts.stack_cap", stack_cap);
       return BH_EOVERFLOW;
     }
-    std::vector<int64_t> ho(2 * (ng + 1));
-    ho[0] = 0;
-    ho[ng + 1] = 0;
+    const size_t budget = c->list_budget ? c->list_budget : LIST_BUDGET;
+    ho.assign(2 * (ng + 1), 0);
+    int64_t maxc = 0, maxk = 0, bc = 0, bk = 0;
+    size_t bytes = 0;
+    batches.push_back({0, 0});
     for (int64_t g = 0; g < ng; g++) {
       ho[g + 1] = ho[g] + hc[g];
       ho[ng + 1 + g + 1] = ho[ng + 1 + g] + hc[ng + g];
       const int64_t tg = gt0 ? t->h_gtab[g + 1] - t->h_gtab[g] : std::min<int64_t>(gs, m - g * gs);
       cell_int += hc[g] * tg;
       pair_int += hc[2 * ng + g] * tg;
+      const size_t gb = hc[g] * sizeof(CellEnt) + hc[ng + g] * sizeof(ChunkEnt);
+      if (g > batches.back().g0 && bytes + gb > budget) {
+        batches.back().g1 = g;
+        batches.push_back({g, 0});
+        maxc = std::max(maxc, bc);
+        maxk = std::max(maxk, bk);
+        bytes = 0;
+        bc = bk = 0;
+      }
+      bytes += gb;
+      bc += hc[g];
+      bk += hc[ng + g];
     }
+    batches.back().g1 = ng;
+    maxc = std::max(maxc, bc);
+    maxk = std::max(maxk, bk);
     if (!tgt) pair_int -= m;  // each target's own pair contributes nothing
     ntc = ho[ng];
     ntk = ho[2 * ng + 1];
-    BH_HIP(hipMemcpyAsync(offs, ho.data(), 2 * (ng + 1) * 8, hipMemcpyHostToDevice, c->stream));
-    BH_HIP(hipStreamSynchronize(c->stream));  // ho is a temporary
-    clist = s.get<CellEnt>((size_t)std::max<int64_t>(ntc, 1));
-    klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(ntk, 1));
+    clist = s.get<CellEnt>((size_t)std::max<int64_t>(maxc, 1));
+    klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(maxk, 1));
   }
   if (!clist || !klist) return BH_ENOMEM;
-  {
-    ProfScope p(c, "k_group_walk_fill");
-    if (t->dim == 3)
-      hipLaunchKernelGGL((k_group_walk<3, true>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
-                         t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr, offs,
-                         offs + ng + 1, clist, klist, err, groot);
-    else
-      hipLaunchKernelGGL((k_group_walk<2, true>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
-                         t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr, offs,
-                         offs + ng + 1, clist, klist, err, groot);
-    BH_LAUNCH_CHECK();
-  }
+  uint64_t* keys = s.get<uint64_t>(ng);
+  uint64_t* keys2 = s.get<uint64_t>(ng);
+  uint32_t* idx = s.get<uint32_t>(ng);
+  uint32_t* idx2 = s.get<uint32_t>(ng);
+  if (!keys || !keys2 || !idx || !idx2) return BH_ENOMEM;
   const float eps2 = (float)(eps * eps);
   const bool zeps = !(eps2 > 0.f);
   const bool self = tgt == nullptr;
-  const int32_t* order = nullptr;
-  {
-    // longest-first dispatch order of the groups
-    ProfScope p(c, "k_group_order");
-    uint64_t* keys = s.get<uint64_t>(ng);
-    uint64_t* keys2 = s.get<uint64_t>(ng);
-    uint32_t* idx = s.get<uint32_t>(ng);
-    uint32_t* idx2 = s.get<uint32_t>(ng);
-    if (!keys || !keys2 || !idx || !idx2) return BH_ENOMEM;
-    hipLaunchKernelGGL(k_group_cost_keys, dim3((unsigned)cdiv(ng, 256)), dim3(256), 0, c->stream, ng, cnt,
-                       cnt + 2 * ng, keys, idx);
-    uint64_t* ko;
-    uint32_t* vo;
-    int rc = radix_sort(c, keys, idx, ng, 32, &ko, &vo, keys2, idx2);
-    if (rc) return rc;
-    order = reinterpret_cast<const int32_t*>(vo);
-  }
-  {
-    ProfScope p(c, "k_group_eval_f32");
-    const int64_t* coff = offs;
-    const int64_t* koff = offs + ng + 1;
-    if (gs == 64) {
-      if (ch == 16) launch_eval<1, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-      else if (ch == 32) launch_eval<1, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-      else launch_eval<1, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-    } else {
-      if (ch == 16) launch_eval<2, 16>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-      else if (ch == 32) launch_eval<2, 32>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-      else launch_eval<2, 64>(c, ng, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+  std::vector<int64_t> rel;
+  for (const Batch& b : batches) {
+    const int64_t nb = b.g1 - b.g0;
+    if (!plan) {
+      // offsets of this batch's groups, relative to its first list entries
+      rel.resize(2 * (nb + 1));
+      for (int64_t g = b.g0; g <= b.g1; g++) {
+        rel[g - b.g0] = ho[g] - ho[b.g0];
+        rel[nb + 1 + g - b.g0] = ho[ng + 1 + g] - ho[ng + 1 + b.g0];
+      }
+      BH_HIP(hipMemcpyAsync(offs + b.g0, rel.data(), (nb + 1) * 8, hipMemcpyHostToDevice, c->stream));
+      BH_HIP(hipMemcpyAsync(offs + ng + 1 + b.g0, rel.data() + nb + 1, (nb + 1) * 8, hipMemcpyHostToDevice,
+                            c->stream));
+      BH_HIP(hipStreamSynchronize(c->stream));  // rel is reused by the next batch
     }
-    BH_LAUNCH_CHECK();
+    {
+      ProfScope p(c, "k_group_walk_fill");
+      if (t->dim == 3)
+        hipLaunchKernelGGL((k_group_walk<3, true>), dim3((unsigned)nb), dim3(64), lds, c->stream, t->nodes,
+                           t->center, t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
+                           offs, offs + ng + 1, clist, klist, err, groot, b.g0);
+      else
+        hipLaunchKernelGGL((k_group_walk<2, true>), dim3((unsigned)nb), dim3(64), lds, c->stream, t->nodes,
+                           t->center, t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
+                           offs, offs + ng + 1, clist, klist, err, groot, b.g0);
+      BH_LAUNCH_CHECK();
+    }
+    const int32_t* order = nullptr;
+    {
+      // longest-first dispatch order of the batch's groups
+      ProfScope p(c, "k_group_order");
+      hipLaunchKernelGGL(k_group_cost_keys, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, c->stream, b.g0, nb, cnt,
+                         cnt + 2 * ng, keys, idx);
+      uint64_t* ko;
+      uint32_t* vo;
+      int rc = radix_sort(c, keys, idx, nb, 32, &ko, &vo, keys2, idx2);
+      if (rc) return rc;
+      order = reinterpret_cast<const int32_t*>(vo);
+    }
+    {
+      ProfScope p(c, "k_group_eval_f32");
+      const int64_t* coff = offs;
+      const int64_t* koff = offs + ng + 1;
+      if (gs == 64) {
+        if (ch == 16) launch_eval<1, 16>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+        else if (ch == 32) launch_eval<1, 32>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+        else launch_eval<1, 64>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+      } else {
+        if (ch == 16) launch_eval<2, 16>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+        else if (ch == 32) launch_eval<2, 32>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+        else launch_eval<2, 64>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+      }
+      BH_LAUNCH_CHECK();
+    }
   }
   if (stats && !plan) {
     stats[0] = cell_int;

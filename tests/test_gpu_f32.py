@@ -116,3 +116,23 @@ def test_api_precision_keyword(native):
     assert rel_err(t.accelerations(theta=0.0, softening=0.01, precision="fp32"), a64)[0] < 2e-6
     with pytest.raises(ValueError, match="precision"):
         t.accelerations(precision="fp16")
+
+
+def test_list_batches_give_identical_results(native):
+    """A small list budget splits the groups into many batches (each filled,
+    ordered and evaluated on its own); every group's lists and sums are the
+    same, so the results are bit-identical to one batch."""
+    x, m = workloads.plummer(30_000, seed=12)
+    rng = np.random.default_rng(3)
+    tg = rng.uniform(-3, 3, (7000, 3))
+    t = native.DeviceTree.build(x, m, 32)
+    c = native.ctx()
+    full = t.accel_f32(None, 0.5, 0.01, 1.0)
+    full_t = t.accel_f32(tg, 0.5, 0.01, 1.0)
+    try:
+        for budget in (1 << 20, 200_000, 1):
+            c.set_list_budget(budget)
+            np.testing.assert_array_equal(t.accel_f32(None, 0.5, 0.01, 1.0).view(np.uint64), full.view(np.uint64))
+            np.testing.assert_array_equal(t.accel_f32(tg, 0.5, 0.01, 1.0).view(np.uint64), full_t.view(np.uint64))
+    finally:
+        c.set_list_budget(0)
