@@ -129,14 +129,18 @@ def f32_options(args, best=None):
 
 
 def roofline_f32(prof, steps, stats):
+    """FLOP rate of the evaluation kernel over one step's groups (one launch
+    per step unless the lists were split into batches)."""
     launches, kms = prof.get("k_group_eval_f32", (0, 0.0))
-    kavg = kms / max(launches, 1)
+    kstep = kms / max(steps, 1)
     flops = FLOPS_PER_INTERACTION * int(stats[1]) + FLOPS_PER_QUAD_CELL * int(stats[0])
-    achieved = flops / (kavg * 1e-3) / 1e12 if kavg > 0 else None
+    achieved = flops / (kstep * 1e-3) / 1e12 if kstep > 0 else None
     return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS if achieved else None, "traffic": None,
-            "kernel": "k_group_eval_f32", "kernel_avg_ms": kavg, "flops_per_launch": flops,
-            "interactions_per_launch": int(stats[0] + stats[1])}
+            "kernel": "k_group_eval_f32", "kernel_avg_ms": kms / max(launches, 1),
+            "launches_per_step": launches / max(steps, 1), "kernel_ms_per_step": kstep,
+            "flops_per_step": flops, "cell_interactions_per_step": int(stats[0]),
+            "pair_interactions_per_step": int(stats[1]), "interactions_per_launch": int(stats[0] + stats[1])}
 
 
 def cpu_baseline(x, m, theta, eps, leaf, budget_s):
@@ -432,7 +436,7 @@ def workload_diskhalo(args, env):
         acc = rel_err(out_t.cpu().numpy()[sel], ref)
         row = {"theta": theta, "chosen": best, "ms": ms, "particle_accels_per_s": N / (ms * 1e-3),
                "interactions_per_s": float(st[0] + st[1]) / (ms * 1e-3), "accuracy": acc,
-               "roofline": roofline_f32(prof, 1, st), "autotune": table,
+               "roofline": roofline_f32(prof, max(1, args.steps // 3), st), "autotune": table,
                "kernels_ms_per_step": {k: v[1] / max(1, args.steps // 3) for k, v in prof.items()}}
         log(f"bench: theta={theta} {best} {ms:.1f} ms/step; {acc}")
         rows.append(row)
@@ -469,7 +473,8 @@ def workload_leapfrog(args, env):
         if e0 is None:
             log("bench: initial energy (FP64 direct potential)")
             t0 = time.perf_counter()
-            e0 = lf.energy()
+            e0 = lf.energy(chunk=1 << 20, progress=lambda k: log(f"bench:   potential {k}/{N} targets, "
+                                                                   f"{time.perf_counter() - t0:.1f} s"))
             log(f"bench: E0 = {e0:.12e} ({time.perf_counter() - t0:.1f} s)")
         env.sync()
         e_a = torch.cuda.Event(enable_timing=True)
@@ -486,7 +491,9 @@ def workload_leapfrog(args, env):
             done += k
             log(f"bench: {prec} {done}/{steps} steps, {total_ms / done:.2f} ms/step")
         log(f"bench: {prec} final energy")
-        e1 = lf.energy()
+        t0 = time.perf_counter()
+        e1 = lf.energy(chunk=1 << 20, progress=lambda k: log(f"bench:   potential {k}/{N} targets, "
+                                                               f"{time.perf_counter() - t0:.1f} s"))
         finals[prec] = lf.pos.cpu().numpy()
         res[prec] = {"ms_per_step": total_ms / steps, "particle_steps_per_s": N * steps / (total_ms * 1e-3),
                      "energy_final": e1, "rel_energy_drift": (e1 - e0) / abs(e0), "sim": lf.info()}

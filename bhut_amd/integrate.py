@@ -60,19 +60,38 @@ class Leapfrog:
     def kinetic_energy(self) -> float:
         return float(0.5 * (self.mass * (self.vel * self.vel).sum(dim=1)).sum())
 
-    def potential_energy(self) -> float:
+    def potential_energy(self, chunk: int = 0, progress=None) -> float:
         """FP64 direct sum of the softened (Plummer) pair potential,
-        W = 1/2 sum_i m_i phi_i."""
+        W = 1/2 sum_i m_i phi_i, optionally over chunks of targets (progress
+        is called with the number of targets done after each chunk)."""
         import torch
         n, d = self.pos.shape
         phi = torch.empty(n, dtype=torch.float64, device=self.pos.device)
-        _native.direct_dev(self.pos.data_ptr(), self.mass.data_ptr(), n, self.pos.data_ptr(), n, d, True,
-                           self.softening, self.G, None, phi.data_ptr())
-        _native.ctx().synchronize()
+        torch.cuda.synchronize(self.pos.device)
+        step = chunk or n
+        for a in range(0, n, step):
+            b = min(n, a + step)
+            if a == 0 and b == n:
+                _native.direct_dev(self.pos.data_ptr(), self.mass.data_ptr(), n, self.pos.data_ptr(), n, d, True,
+                                   self.softening, self.G, None, phi.data_ptr())
+            else:
+                # targets a..b are particles a..b: the self pair is skipped by
+                # zero distance contributing nothing only when softened, so
+                # remove it explicitly below
+                _native.direct_dev(self.pos.data_ptr(), self.mass.data_ptr(), n, self.pos[a:b].data_ptr(), b - a,
+                                   d, False, self.softening, self.G, None, phi[a:b].data_ptr())
+            _native.ctx().synchronize()
+            if progress:
+                progress(b)
+        if chunk and chunk < n:
+            if self.softening > 0:
+                phi += self.G * self.mass / self.softening  # the self term -G m_i / eps
+            else:
+                raise ValueError("chunked potential needs softening > 0")
         return float(0.5 * (self.mass * phi).sum())
 
-    def energy(self) -> float:
-        return self.kinetic_energy() + self.potential_energy()
+    def energy(self, chunk: int = 0, progress=None) -> float:
+        return self.kinetic_energy() + self.potential_energy(chunk, progress)
 
     def info(self) -> dict:
         return self.sim.info()
