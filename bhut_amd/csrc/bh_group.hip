@@ -428,37 +428,42 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int EV_WAVES = 4;  // groups (wavefronts) per workgroup
 
+// Work items.  A group's list is a sequence of tiles: its cell tiles (64
+// cells each) then its chunk tiles (64/CH chunks each).  Item j of a group
+// covers tiles [j*ITEM_TILES, (j+1)*ITEM_TILES) of that sequence, so every
+// item costs at most ITEM_TILES tiles and a wavefront's tail is at most one
+// item long.  The split depends only on the group's own list lengths.
+constexpr int ITEM_TILES = 64;
+
+__host__ __device__ __forceinline__ int64_t group_items(int64_t ncell, int64_t nchunk, int cpt) {
+  const int64_t tiles = (ncell + 63) / 64 + (nchunk + cpt - 1) / cpt;
+  const int64_t k = (tiles + ITEM_TILES - 1) / ITEM_TILES;
+  return k > 1 ? k : 1;
+}
+
 // CH: particles per chunk (64/CH chunks fill one 64-slot tile).
 // One LDS tile per wavefront, refilled in place: the wavefront issued every
 // read of the old tile before its writes of the new one, and its LDS
 // operations execute in order.  The next tile's HBM reads are in flight in
-// registers while the current tile is consumed.
+// registers while the current tile is consumed.  The item's compensated
+// sums go to part[(q*64 + lane)*3 + d] as FP64 (sum - compensation).
 template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
-__global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
-    int64_t ngroups, const int32_t* __restrict__ order, int64_t m, int gs, const double4* __restrict__ tpos,
-    const uint32_t* __restrict__ torig, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
+__device__ __forceinline__ void eval_item(
+    int64_t g, int64_t ta, int64_t tb, int lane, float4 (*tile)[64], int64_t m, int gs,
+    const double4* __restrict__ tpos, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
     const double* __restrict__ center, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
-    const ChunkEnt* __restrict__ klist, float eps2, double G, double* __restrict__ out,
-    const int32_t* __restrict__ gt0, const uint32_t* __restrict__ err) {
-  __shared__ float4 tiles[EV_WAVES][3][64];
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-  const int64_t slot = (int64_t)blockIdx.x * EV_WAVES + w;
-  if (slot >= ngroups) return;  // whole wavefront
-  if (__builtin_amdgcn_readfirstlane(*err) != 0u) return;  // lists incomplete
-  const int64_t g = order ? (int64_t)order[slot] : slot;
-  float4 (*tile)[64] = tiles[w];
-  const int lane = threadIdx.x % 64;
+    const ChunkEnt* __restrict__ klist, float eps2, const int32_t* __restrict__ gt0, double* __restrict__ part) {
+  constexpr int CPT = 64 / CH;
   int64_t t0, t1;
   group_range(g, m, gs, gt0, t0, t1);
   const GroupBox B = boxes[g];
   float tx[TPL], ty[TPL], tz[TPL];
-  int64_t ti[TPL];
   float sx[TPL], sy[TPL], sz[TPL], cx[TPL], cy[TPL], cz[TPL], px[TPL], py[TPL], pz[TPL];
 #pragma unroll
   for (int q = 0; q < TPL; q++) {
-    ti[q] = t0 + lane + 64 * q;
-    const int64_t i = ti[q] < t1 ? ti[q] : t0;
+    const int64_t ti = t0 + lane + 64 * q;
+    const int64_t i = ti < t1 ? ti : t0;
     if (SELF) {
       // built exactly like the source copy of the same particle, so its own
       // pair has dx == 0 and contributes nothing
@@ -475,29 +480,32 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     }
     sx[q] = sy[q] = sz[q] = cx[q] = cy[q] = cz[q] = px[q] = py[q] = pz[q] = 0.f;
   }
-
-  // ---- accepted cells: tiles of 64 entries
   const int64_t c0 = coff[g], c1 = coff[g + 1];
-  if (c1 > c0) {
+  const int64_t ctiles = (c1 - c0 + 63) / 64;
+
+  // ---- accepted cells of tiles [ta, min(tb, ctiles))
+  if (ta < ctiles) {
+    const int64_t cs = c0 + 64 * ta;
+    const int64_t ce = tb < ctiles ? c0 + 64 * tb : c1;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 na = z4, nb = z4, nc = z4;
-    if (c0 + lane < c1) {
-      na = clist[c0 + lane].a;
-      nb = clist[c0 + lane].b;
-      nc = clist[c0 + lane].c;
+    if (cs + lane < ce) {
+      na = clist[cs + lane].a;
+      nb = clist[cs + lane].b;
+      nc = clist[cs + lane].c;
     }
-    for (int64_t e = c0; e < c1; e += 64) {
+    for (int64_t e = cs; e < ce; e += 64) {
       wave_sync();
       tile[0][lane] = na;
       tile[1][lane] = nb;
       tile[2][lane] = nc;
       wave_sync();
-      if (e + 64 + lane < c1) {
+      if (e + 64 + lane < ce) {
         na = clist[e + 64 + lane].a;
         nb = clist[e + 64 + lane].b;
         nc = clist[e + 64 + lane].c;
       }
-      const int cnt = (int)(c1 - e < 64 ? c1 - e : 64);
+      const int cnt = (int)(ce - e < 64 ? ce - e : 64);
       for (int k = 0; k < cnt; k++) {
         const float4 a = tile[0][k], b = tile[1][k], c = tile[2][k];
 #pragma unroll
@@ -507,11 +515,14 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     }
   }
 
-  // ---- leaf chunks: tiles of 64/CH chunks, lane l holds particle l%CH of
-  // chunk l/CH (padding slots carry zero mass far away)
-  constexpr int CPT = 64 / CH;
+  // ---- leaf chunks of tiles [max(ta, ctiles), tb) - ctiles: lane l holds
+  // particle l%CH of chunk l/CH (padding slots carry zero mass far away)
   const int64_t k0 = koff[g], k1 = koff[g + 1];
-  if (k1 > k0) {
+  const int64_t ka0 = ta > ctiles ? ta - ctiles : 0;
+  const int64_t kb0 = tb - ctiles;
+  const int64_t ks = k0 + CPT * ka0;
+  const int64_t ke = k0 + CPT * kb0 < k1 ? k0 + CPT * kb0 : k1;
+  if (kb0 > 0 && ks < ke) {
     // Two-stage pipeline over tiles: the chunk entries of tile i+2 and the
     // sources of tile i+1 are in flight while tile i is consumed; the shift
     // is applied only when a tile is written to LDS, so no load result is
@@ -519,30 +530,30 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     const int sub = lane / CH, off = lane % CH;
     const float4* kl4 = reinterpret_cast<const float4*>(klist);  // ChunkEnt = 2 x 16 B
     // raw chunk fields: shift xyz + start bits in .w, count bits in .x
-    auto load_chunk = [&](int64_t e, float4& ka, float4& kb) {
-      const int64_t ce = e + sub;
-      if (ce < k1) {
-        ka = kl4[2 * ce];
-        kb = kl4[2 * ce + 1];
+    auto load_chunk = [&](int64_t e, float4& kx, float4& ky) {
+      const int64_t ce2 = e + sub;
+      if (ce2 < ke) {
+        kx = kl4[2 * ce2];
+        ky = kl4[2 * ce2 + 1];
       } else {
-        ka = make_float4(0.f, 0.f, 0.f, 0.f);
-        kb = make_float4(0.f, 0.f, 0.f, 0.f);  // count 0: every slot is padding
+        kx = make_float4(0.f, 0.f, 0.f, 0.f);
+        ky = make_float4(0.f, 0.f, 0.f, 0.f);  // count 0: every slot is padding
       }
     };
-    auto src_index = [&](const float4& ka, const float4& kb) -> int {
-      return off < __float_as_int(kb.x) ? __float_as_int(ka.w) + off : -1;
+    auto src_index = [&](const float4& kx, const float4& ky) -> int {
+      return off < __float_as_int(ky.x) ? __float_as_int(kx.w) + off : -1;
     };
     auto load_src = [&](int kidx) -> float4 {
-      float4 s = make_float4(1e15f, 1e15f, 1e15f, 0.f);
-      if (kidx >= 0) s = src32[kidx];
-      return s;
+      float4 v = make_float4(1e15f, 1e15f, 1e15f, 0.f);
+      if (kidx >= 0) v = src32[kidx];
+      return v;
     };
     float4 ka_cur, kb_cur, ka_nxt, kb_nxt;
-    load_chunk(k0, ka_cur, kb_cur);
+    load_chunk(ks, ka_cur, kb_cur);
     int idx_cur = src_index(ka_cur, kb_cur);
     float4 src_cur = load_src(idx_cur);
-    load_chunk(k0 + CPT, ka_nxt, kb_nxt);
-    for (int64_t e = k0; e < k1; e += CPT) {
+    load_chunk(ks + CPT, ka_nxt, kb_nxt);
+    for (int64_t e = ks; e < ke; e += CPT) {
       wave_sync();
       tile[0][lane] = idx_cur >= 0 ? make_float4(src_cur.x + ka_cur.x, src_cur.y + ka_cur.y,
                                                  src_cur.z + ka_cur.z, src_cur.w)
@@ -553,87 +564,147 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
       idx_cur = src_index(ka_cur, kb_cur);
       src_cur = load_src(idx_cur);
       load_chunk(e + 2 * CPT, ka_nxt, kb_nxt);
-      const int nslots = (int)((k1 - e < CPT ? k1 - e : CPT) * CH);
+      const int nslots = (int)((ke - e < CPT ? ke - e : CPT) * CH);
 #pragma unroll 4
       for (int k = 0; k < nslots; k++) {
-        const float4 s = tile[0][k];
+        const float4 v = tile[0][k];
 #pragma unroll
-        for (int q = 0; q < TPL; q++) p2p<ZEPS>(s, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
+        for (int q = 0; q < TPL; q++) p2p<ZEPS>(v, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
       }
       flush<TPL, KAHAN>(px, py, pz, sx, sy, sz, cx, cy, cz);
     }
   }
 #pragma unroll
   for (int q = 0; q < TPL; q++) {
-    if (ti[q] < t1) {
-      const int64_t o = torig[ti[q]];
-      out[o * dim] = G * ((double)sx[q] - (double)cx[q]);
-      out[o * dim + 1] = G * ((double)sy[q] - (double)cy[q]);
-      if (dim == 3) out[o * dim + 2] = G * ((double)sz[q] - (double)cz[q]);
-    }
+    double* o = part + (int64_t)(q * 64 + lane) * 3;
+    o[0] = (double)sx[q] - (double)cx[q];
+    o[1] = (double)sy[q] - (double)cy[q];
+    o[2] = (double)sz[q] - (double)cz[q];
   }
 }
 
-// Longest-first schedule: key = complement of the group's estimated work
-// (particle pairs + 3 x accepted cells) so an ascending sort puts the most
-// expensive groups at the front of the dispatch order.
-__global__ void k_group_cost_keys(int64_t g0, int64_t nb, const int64_t* __restrict__ ccount,
-                                  const int64_t* __restrict__ pcount, uint64_t* __restrict__ keys,
-                                  uint32_t* __restrict__ idx) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  const int64_t g = g0 + i;
-  const uint64_t cost = (uint64_t)pcount[g] + 3ull * (uint64_t)ccount[g];
-  keys[i] = 0xffffffffull - (cost < 0xffffffffull ? cost : 0xffffffffull);
-  idx[i] = (uint32_t)g;
+// Persistent evaluation: as many workgroups as fit on the device at once;
+// each wavefront takes the next work item from a device counter until none
+// is left.  Items are numbered group by group (ioff[g] = first item of group
+// g, ioff[ng] = number of items).  *next must be zero at launch.
+template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
+__global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
+    int64_t g0, int64_t ngroups, const int32_t* __restrict__ ioff, int64_t m, int gs,
+    const double4* __restrict__ tpos, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
+    const double* __restrict__ center, int dim, const GroupBox* __restrict__ boxes,
+    const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
+    const ChunkEnt* __restrict__ klist, float eps2, const int32_t* __restrict__ gt0,
+    const uint32_t* __restrict__ err, uint32_t* __restrict__ next, double* __restrict__ part) {
+  constexpr int CPT = 64 / CH;
+  __shared__ float4 tiles[EV_WAVES][3][64];
+  if (__builtin_amdgcn_readfirstlane(*err) != 0u) return;  // lists incomplete
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int lane = threadIdx.x % 64;
+  const int64_t nitems = ioff[ngroups];
+  for (;;) {
+    uint32_t sl = 0;
+    if (lane == 0) sl = atomicAdd(next, 1u);
+    const int64_t item = (int64_t)__builtin_amdgcn_readfirstlane(sl);
+    if (item >= nitems) break;  // whole wavefront
+    // the item's group: last gi with ioff[gi] <= item
+    int64_t lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) / 2;
+      if (ioff[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const int64_t g = g0 + lo;
+    const int64_t j = item - ioff[lo];
+    const int64_t tiles_g = (coff[g + 1] - coff[g] + 63) / 64 + (koff[g + 1] - koff[g] + CPT - 1) / CPT;
+    const int64_t ta = j * ITEM_TILES;
+    const int64_t tb = ta + ITEM_TILES < tiles_g ? ta + ITEM_TILES : tiles_g;
+    eval_item<TPL, CH, ZEPS, SELF, KAHAN, QUAD>(g, ta, tb, lane, tiles[w], m, gs, tpos, src32, pleaf, center, dim,
+                                                boxes, coff, clist, koff, klist, eps2, gt0,
+                                                part + item * (int64_t)(64 * TPL * 3));
+  }
 }
 
-// Exclusive scans of the per-group cell and chunk counts into list offsets,
-// on the device (one workgroup), with the capacity check of the fixed-size
-// lists: totals above capacity set bit 1 of *err.
-__global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* __restrict__ cnt,
-                                                     int64_t* __restrict__ offs, int64_t cap_c, int64_t cap_k,
+// Accelerations from the items' partial sums, added in item order (so the
+// result does not depend on which wavefront ran which item).
+__global__ __launch_bounds__(256) void k_item_reduce(int64_t g0, int64_t ngroups, const int32_t* __restrict__ ioff,
+                                                     int64_t m, int gs, const int32_t* __restrict__ gt0,
+                                                     const double* __restrict__ part, int tpg,
+                                                     const uint32_t* __restrict__ torig, int dim, double G,
+                                                     const uint32_t* __restrict__ err, double* __restrict__ out) {
+  if (*err != 0u) return;
+  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (x >= ngroups * tpg) return;
+  const int64_t lg = x / tpg, l = x % tpg;
+  int64_t t0, t1;
+  group_range(g0 + lg, m, gs, gt0, t0, t1);
+  if (t0 + l >= t1) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int64_t it = ioff[lg]; it < ioff[lg + 1]; it++) {
+    const double* p = part + (it * tpg + l) * 3;
+    a0 = a0 + p[0];
+    a1 = a1 + p[1];
+    a2 = a2 + p[2];
+  }
+  const int64_t o = torig[t0 + l];
+  out[o * dim] = G * a0;
+  out[o * dim + 1] = G * a1;
+  if (dim == 3) out[o * dim + 2] = G * a2;
+}
+
+// Exclusive scans of the per-group cell and chunk counts into list offsets
+// and of the per-group item counts into item offsets, on the device (one
+// workgroup), with the capacity check of the fixed-size lists and partial
+// buffers: totals above capacity set bit 1 of *err.
+__global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* __restrict__ cnt, int cpt,
+                                                     int64_t* __restrict__ offs, int32_t* __restrict__ ioff,
+                                                     int64_t cap_c, int64_t cap_k, int64_t cap_items,
                                                      uint32_t* __restrict__ err, int64_t* __restrict__ need) {
   constexpr int IT = 8;
-  __shared__ int64_t wsum[2][16];
+  __shared__ int64_t wsum[3][16];
   const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
   int64_t* coff = offs;
   int64_t* koff = offs + ng + 1;
-  int64_t carry_c = 0, carry_k = 0;
+  int64_t carry_c = 0, carry_k = 0, carry_i = 0;
   for (int64_t base = 0; base < ng; base += 1024 * IT) {
-    int64_t vc[IT], vk[IT];
-    int64_t sc = 0, sk = 0;
+    int64_t vc[IT], vk[IT], vi[IT];
+    int64_t sc = 0, sk = 0, si = 0;
 #pragma unroll
     for (int i = 0; i < IT; i++) {
       const int64_t g = base + (int64_t)tid * IT + i;
       vc[i] = g < ng ? cnt[g] : 0;
       vk[i] = g < ng ? cnt[ng + g] : 0;
+      vi[i] = g < ng ? group_items(vc[i], vk[i], cpt) : 0;
       sc += vc[i];
       sk += vk[i];
+      si += vi[i];
     }
-    int64_t ic = sc, ik = sk;  // inclusive wave scan of the thread sums
+    int64_t ic = sc, ik = sk, ii = si;  // inclusive wave scan of the thread sums
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-      const int64_t yc = __shfl_up(ic, off), yk = __shfl_up(ik, off);
+      const int64_t yc = __shfl_up(ic, off), yk = __shfl_up(ik, off), yi = __shfl_up(ii, off);
       if (lane >= off) {
         ic += yc;
         ik += yk;
+        ii += yi;
       }
     }
     if (lane == 63) {
       wsum[0][w] = ic;
       wsum[1][w] = ik;
+      wsum[2][w] = ii;
     }
     __syncthreads();
-    int64_t pc = carry_c + ic - sc, pk = carry_k + ik - sk;
-    int64_t tc = 0, tk = 0;
+    int64_t pc = carry_c + ic - sc, pk = carry_k + ik - sk, pi = carry_i + ii - si;
+    int64_t tc = 0, tk = 0, tn = 0;
     for (int j = 0; j < 16; j++) {
       if (j < w) {
         pc += wsum[0][j];
         pk += wsum[1][j];
+        pi += wsum[2][j];
       }
       tc += wsum[0][j];
       tk += wsum[1][j];
+      tn += wsum[2][j];
     }
     __syncthreads();  // wsum is rewritten by the next chunk
 #pragma unroll
@@ -642,19 +713,23 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
       if (g < ng) {
         coff[g] = pc;
         koff[g] = pk;
+        ioff[g] = (int32_t)pi;
       }
       pc += vc[i];
       pk += vk[i];
+      pi += vi[i];
     }
     carry_c += tc;
     carry_k += tk;
+    carry_i += tn;
   }
   if (tid == 0) {
     coff[ng] = carry_c;
     koff[ng] = carry_k;
+    ioff[ng] = (int32_t)carry_i;
     need[0] = carry_c;
     need[1] = carry_k;
-    if (carry_c > cap_c || carry_k > cap_k) atomicOr(err, 2u);
+    if (carry_c > cap_c || carry_k > cap_k || carry_i > cap_items) atomicOr(err, 2u);
   }
 }
 
@@ -663,18 +738,44 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
 // unless the context sets another budget.
 constexpr size_t LIST_BUDGET = size_t(12) << 30;
 
+// Workgroups of the persistent evaluation: all that fit at once, but no
+// more than the groups need.
+static unsigned eval_grid(bh_ctx* c, const void* kern, int64_t ng) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> per_cu;
+  int n = 0;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = per_cu.find({kern, c->device});
+    if (it != per_cu.end()) {
+      n = it->second;
+    } else {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 64 * EV_WAVES, 0) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        n = 1;
+      }
+      per_cu[{kern, c->device}] = n;
+    }
+  }
+  const int64_t resident = (int64_t)n * std::max(c->num_cus, 1);
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(ng, EV_WAVES), resident));
+}
+
 template <int TPL, int CH>
-static void launch_eval(bh_ctx* c, int64_t ng, const int32_t* order, bool zeps, bool self, bool kahan, bool quad,
-                        int64_t m, int gs, const double4* tpos, const uint32_t* torig, const bh_tree* t,
-                        const GroupBox* boxes, const int64_t* coff, const CellEnt* clist, const int64_t* koff,
-                        const ChunkEnt* klist, float eps2, double G, double* out, const int32_t* gt0,
-                        const uint32_t* err) {
-#define BH_EV(Z, S, K, Q)                                                                                       \
-  hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3((unsigned)cdiv(ng, EV_WAVES)), dim3(64 * EV_WAVES), 0, \
-                     c->stream, ng, order, m, gs, tpos, torig, t->spart32, t->pleaf, t->center, t->dim, boxes, coff,  \
-                     clist, koff, klist, eps2, G, out, gt0, err)
+static void launch_eval(bh_ctx* c, int64_t g0, int64_t ng, const int32_t* ioff, bool zeps, bool self, bool kahan,
+                        bool quad, int64_t m, int gs, const double4* tpos, const bh_tree* t, const GroupBox* boxes,
+                        const int64_t* coff, const CellEnt* clist, const int64_t* koff, const ChunkEnt* klist,
+                        float eps2, const int32_t* gt0, const uint32_t* err, uint32_t* next, double* part,
+                        int64_t items_bound) {
+#define BH_EV(Z, S, K, Q)                                                                                          \
+  {                                                                                                                \
+    const void* kern = reinterpret_cast<const void*>(&k_group_eval<TPL, CH, Z, S, K, Q>);                         \
+    hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3(eval_grid(c, kern, items_bound)),                 \
+                       dim3(64 * EV_WAVES), 0, c->stream, g0, ng, ioff, m, gs, tpos, t->spart32, t->pleaf,        \
+                       t->center, t->dim, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part);            \
+  }
 #define BH_EV_Q(Z, S, K) \
-  if (quad) BH_EV(Z, S, K, true); else BH_EV(Z, S, K, false)
+  if (quad) BH_EV(Z, S, K, true) else BH_EV(Z, S, K, false)
 #define BH_EV_K(Z, S) \
   if (kahan) { BH_EV_Q(Z, S, true); } else { BH_EV_Q(Z, S, false); }
 #define BH_EV_S(Z) \
@@ -790,26 +891,31 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     BH_LAUNCH_CHECK();
   }
   // Lists are materialised for batches of consecutive groups whose lists
-  // fit LIST_BUDGET bytes (one batch unless the problem is very large);
-  // each batch is filled, ordered longest first and evaluated in turn.
+  // fit the list budget (one batch unless the problem is very large); each
+  // batch is filled, split into work items, evaluated and reduced in turn.
+  const int cpt = 64 / ch;
   struct Batch {
-    int64_t g0, g1;
+    int64_t g0, g1, items;
   };
   std::vector<Batch> batches;
-  std::vector<int64_t> ho;
+  std::vector<int64_t> ho, hc;
   CellEnt* clist;
   ChunkEnt* klist;
-  int64_t cell_int = 0, pair_int = 0, ntc = 0, ntk = 0;
+  int32_t* ioff = s.get<int32_t>(ng + 1);
+  if (!ioff) return BH_ENOMEM;
+  int64_t cell_int = 0, pair_int = 0, ntc = 0, ntk = 0, max_items = 0;
   if (plan) {
-    // offsets scanned on the device into lists of the planned capacity
-    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, offs, plan->cap_cells,
-                       plan->cap_chunks, err, plan->need);
+    // offsets scanned on the device into lists of the planned capacity; the
+    // item count is bounded by the capacities
+    max_items = 2 * ng + (plan->cap_cells / 64 + plan->cap_chunks / cpt + 2 * ng) / ITEM_TILES + 1;
+    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, cpt, offs, ioff, plan->cap_cells,
+                       plan->cap_chunks, max_items, err, plan->need);
     BH_LAUNCH_CHECK();
     clist = s.get<CellEnt>((size_t)std::max<int64_t>(plan->cap_cells, 1));
     klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(plan->cap_chunks, 1));
-    batches.push_back({0, ng});
+    batches.push_back({0, ng, max_items});
   } else {
-    std::vector<int64_t> hc(3 * ng);
+    hc.resize(3 * ng);
     uint32_t herr = 0;
     BH_HIP(hipMemcpyAsync(hc.data(), cnt, 3 * ng * 8, hipMemcpyDeviceToHost, c->stream));
     BH_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
@@ -820,9 +926,9 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     }
     const size_t budget = c->list_budget ? c->list_budget : LIST_BUDGET;
     ho.assign(2 * (ng + 1), 0);
-    int64_t maxc = 0, maxk = 0, bc = 0, bk = 0;
+    int64_t maxc = 0, maxk = 0, bc = 0, bk = 0, bi = 0;
     size_t bytes = 0;
-    batches.push_back({0, 0});
+    batches.push_back({0, 0, 0});
     for (int64_t g = 0; g < ng; g++) {
       ho[g + 1] = ho[g] + hc[g];
       ho[ng + 1 + g + 1] = ho[ng + 1 + g] + hc[ng + g];
@@ -832,19 +938,23 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       const size_t gb = hc[g] * sizeof(CellEnt) + hc[ng + g] * sizeof(ChunkEnt);
       if (g > batches.back().g0 && bytes + gb > budget) {
         batches.back().g1 = g;
-        batches.push_back({g, 0});
+        batches.back().items = bi;
+        batches.push_back({g, 0, 0});
         maxc = std::max(maxc, bc);
         maxk = std::max(maxk, bk);
         bytes = 0;
-        bc = bk = 0;
+        bc = bk = bi = 0;
       }
       bytes += gb;
       bc += hc[g];
       bk += hc[ng + g];
+      bi += group_items(hc[g], hc[ng + g], cpt);
     }
     batches.back().g1 = ng;
+    batches.back().items = bi;
     maxc = std::max(maxc, bc);
     maxk = std::max(maxk, bk);
+    for (const Batch& b : batches) max_items = std::max(max_items, b.items);
     if (!tgt) pair_int -= m;  // each target's own pair contributes nothing
     ntc = ho[ng];
     ntk = ho[2 * ng + 1];
@@ -852,28 +962,34 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(maxk, 1));
   }
   if (!clist || !klist) return BH_ENOMEM;
-  uint64_t* keys = s.get<uint64_t>(ng);
-  uint64_t* keys2 = s.get<uint64_t>(ng);
-  uint32_t* idx = s.get<uint32_t>(ng);
-  uint32_t* idx2 = s.get<uint32_t>(ng);
-  if (!keys || !keys2 || !idx || !idx2) return BH_ENOMEM;
+  const int tpg = gs;  // targets per group = 64 * TPL
+  double* part = s.get<double>((size_t)std::max<int64_t>(max_items, 1) * tpg * 3);
+  uint32_t* next = s.get<uint32_t>(1);
+  if (!part || !next) return BH_ENOMEM;
   const float eps2 = (float)(eps * eps);
   const bool zeps = !(eps2 > 0.f);
   const bool self = tgt == nullptr;
   std::vector<int64_t> rel;
+  std::vector<int32_t> reli;
   for (const Batch& b : batches) {
     const int64_t nb = b.g1 - b.g0;
     if (!plan) {
-      // offsets of this batch's groups, relative to its first list entries
+      // offsets of this batch's groups and items, relative to its first
+      // list entries and first item
       rel.resize(2 * (nb + 1));
+      reli.resize(nb + 1);
+      int64_t it = 0;
       for (int64_t g = b.g0; g <= b.g1; g++) {
         rel[g - b.g0] = ho[g] - ho[b.g0];
         rel[nb + 1 + g - b.g0] = ho[ng + 1 + g] - ho[ng + 1 + b.g0];
+        reli[g - b.g0] = (int32_t)it;
+        if (g < b.g1) it += group_items(hc[g], hc[ng + g], cpt);
       }
       BH_HIP(hipMemcpyAsync(offs + b.g0, rel.data(), (nb + 1) * 8, hipMemcpyHostToDevice, c->stream));
       BH_HIP(hipMemcpyAsync(offs + ng + 1 + b.g0, rel.data() + nb + 1, (nb + 1) * 8, hipMemcpyHostToDevice,
                             c->stream));
-      BH_HIP(hipStreamSynchronize(c->stream));  // rel is reused by the next batch
+      BH_HIP(hipMemcpyAsync(ioff, reli.data(), (nb + 1) * 4, hipMemcpyHostToDevice, c->stream));
+      BH_HIP(hipStreamSynchronize(c->stream));  // rel, reli are reused by the next batch
     }
     {
       ProfScope p(c, "k_group_walk_fill");
@@ -887,31 +1003,27 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
                            offs, offs + ng + 1, clist, klist, err, groot, b.g0);
       BH_LAUNCH_CHECK();
     }
-    const int32_t* order = nullptr;
     {
-      // longest-first dispatch order of the batch's groups
-      ProfScope p(c, "k_group_order");
-      hipLaunchKernelGGL(k_group_cost_keys, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, c->stream, b.g0, nb, cnt,
-                         cnt + 2 * ng, keys, idx);
-      uint64_t* ko;
-      uint32_t* vo;
-      int rc = radix_sort(c, keys, idx, nb, 32, &ko, &vo, keys2, idx2);
+      int rc = fill_u32(c, next, 0u, 1);
       if (rc) return rc;
-      order = reinterpret_cast<const int32_t*>(vo);
-    }
-    {
       ProfScope p(c, "k_group_eval_f32");
       const int64_t* coff = offs;
       const int64_t* koff = offs + ng + 1;
       if (gs == 64) {
-        if (ch == 16) launch_eval<1, 16>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-        else if (ch == 32) launch_eval<1, 32>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-        else launch_eval<1, 64>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+        if (ch == 16) launch_eval<1, 16>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
+        else if (ch == 32) launch_eval<1, 32>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
+        else launch_eval<1, 64>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
       } else {
-        if (ch == 16) launch_eval<2, 16>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-        else if (ch == 32) launch_eval<2, 32>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
-        else launch_eval<2, 64>(c, nb, order, zeps, self, kahan, quad, m, gs, tpos, torig, t, boxes, coff, clist, koff, klist, eps2, G, out, gt0, err);
+        if (ch == 16) launch_eval<2, 16>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
+        else if (ch == 32) launch_eval<2, 32>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
+        else launch_eval<2, 64>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
       }
+      BH_LAUNCH_CHECK();
+    }
+    {
+      ProfScope p(c, "k_item_reduce");
+      hipLaunchKernelGGL(k_item_reduce, dim3((unsigned)cdiv(nb * tpg, 256)), dim3(256), 0, c->stream, b.g0, nb, ioff,
+                         m, gs, gt0, part, tpg, torig, t->dim, G, err, out);
       BH_LAUNCH_CHECK();
     }
   }
