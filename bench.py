@@ -17,6 +17,8 @@ JSON line; the default run prints only the headline line):
              leaf / group size / stack depth autotuned per theta
   leapfrog   4M Plummer, 100 KDK steps, refit every step, rebuild every 10,
              steps captured in HIP graphs; FP32 vs FP64 forces
+  copy       three 256 MiB device-to-device copies: the known-size kernel that
+             calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters
 """
 
 from __future__ import annotations
@@ -49,7 +51,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=["plummer", "ensemble", "diskhalo", "leapfrog"], default="plummer")
+    p.add_argument("--workload", choices=["plummer", "ensemble", "diskhalo", "leapfrog", "copy"], default="plummer")
     p.add_argument("--n", type=int, default=0, help="particles (0: the workload's own size)")
     p.add_argument("--theta", type=float, default=0.5)
     p.add_argument("--softening", type=float, default=0.01)
@@ -520,6 +522,22 @@ def workload_leapfrog(args, env):
     return line
 
 
+def workload_copy(args, env):
+    torch = env.torch
+    nbytes = 256 << 20
+    a = torch.full((nbytes // 4,), 1.0, dtype=torch.float32, device=env.dev)
+    b = torch.empty_like(a)
+    env.sync()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        b.copy_(a)
+    env.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / 3
+    return {"metric": "device-to-device copy GB/s (counter calibration)", "value": 2 * nbytes / (ms * 1e-3) / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 0, "ms_per_step": ms, "higher_is_better": True,
+            "config": {"workload": "copy", "bytes": nbytes}, "roofline": None, "cpu_baseline": None}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -528,7 +546,7 @@ def main():
         sys.exit(2)
     env = Env()
     fn = {"plummer": workload_plummer, "ensemble": workload_ensemble, "diskhalo": workload_diskhalo,
-          "leapfrog": workload_leapfrog}[args.workload]
+          "leapfrog": workload_leapfrog, "copy": workload_copy}[args.workload]
     line = fn(args, env)
     print(json.dumps(line), flush=True)
 
