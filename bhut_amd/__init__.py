@@ -5,8 +5,8 @@ bhut/__init__.py:8-11); the tree build, refit and traversal run in
 hand-written HIP kernels (libbhut_hip.so, C ABI in include/bhut_hip.h).
 """
 
-from .api import Tree, accelerations
+from .api import Tree, accelerations, potential
 from .batched import Forest, accelerations_batched
 
 __version__ = "0.1.0"
-__all__ = ["Tree", "accelerations", "Forest", "accelerations_batched"]
+__all__ = ["Tree", "accelerations", "potential", "Forest", "accelerations_batched"]

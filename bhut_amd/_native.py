@@ -64,6 +64,8 @@ _SIGS = {
     "bh_accel_f32_self_dev": (ctypes.c_int, [vp, dbl, dbl, dbl, vp, vp, vp]),
     "bh_accel_f32_targets": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp, vp, vp]),
     "bh_accel_f32_targets_dev": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp, vp, vp]),
+    "bh_potential": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp]),
+    "bh_potential_dev": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp]),
     "bh_interaction_lists": (ctypes.c_int, [vp, vp, i64, dbl, vp, vp, i64, ctypes.POINTER(i64)]),
     "bh_counters": (ctypes.c_int, [vp, vp, i64, dbl, vp]),
     "bh_direct": (ctypes.c_int, [vp, vp, vp, i64, vp, i64, i32, i32, dbl, dbl, vp, vp]),
@@ -352,6 +354,20 @@ class DeviceTree:
                 check(load().bh_accel_f32_targets(self.handle, ptr(tgt), len(tgt), float(theta),
                                                   float(softening), float(G), op, ptr(out), ptr(st)))
         return (out, st) if stats else out
+
+    def potential(self, targets, theta: float, softening: float, G: float) -> np.ndarray:
+        """Softened potential at the sources (targets None) or at targets."""
+        tgt = None if targets is None else f64c(targets)
+        m = self.n if tgt is None else len(tgt)
+        out = np.empty(m)
+        if m:
+            check(load().bh_potential(self.handle, ptr(tgt), m, float(theta), float(softening), float(G), ptr(out)))
+        return out
+
+    def potential_dev(self, tgt_ptr: int | None, m: int, out_ptr: int, theta: float, softening: float,
+                      G: float) -> None:
+        check(load().bh_potential_dev(self.handle, vp(tgt_ptr) if tgt_ptr else None, int(m), float(theta),
+                                      float(softening), float(G), vp(out_ptr)))
 
     def interaction_lists(self, targets, theta: float):
         """(ptr, ids) of per-target interaction lists in pop order."""

@@ -16,6 +16,7 @@ import numpy as np
 from . import backends
 from .traverse.bh import barnes_hut_accelerations, barnes_hut_accelerations_targets
 from .tree.build import build_tree as _build_tree_impl
+from .tree.build import device_tree_for
 from .tree.node import TreeData
 from .tree.refit import refit_tree, should_refit_vs_rebuild
 
@@ -107,6 +108,22 @@ def evaluate_accelerations(tree: TreeData, positions, masses, theta: float = 0.5
     return barnes_hut_accelerations(tree, positions_np, masses_np, softening_scalar, theta, G, precision=precision)
 
 
+def potential(positions: ArrayLike, masses: ArrayLike, *, theta: float = 0.5, softening: float = 0.0,
+              G: float = 1.0, dim: int = 3, backend: str = "auto", leaf_size: int = 32) -> np.ndarray:
+    """Softened gravitational potential phi_i = -G sum_j m_j / sqrt(r_ij^2 + eps^2)
+    (self excluded) by the plain Barnes-Hut walk.  bhut's docs call
+    `bhut.potential(positions, masses, theta=0.5)` (docs/getting-started.md:36)
+    but the package does not define it; the arguments and validation here
+    follow accelerations().  Total potential energy = 0.5 * sum(m * phi)."""
+    backends.get_namespace(positions, backend)
+    positions, masses, softening = _validate_accelerations_inputs(
+        positions, masses, dim=dim, softening=softening, theta=theta, G=G, leaf_size=leaf_size)
+    if not np.isscalar(softening):
+        raise ValueError("Only scalar softening is currently supported")
+    tree = build_tree(positions, masses, leaf_size=leaf_size, backend=backend, dim=dim)
+    return device_tree_for(tree, positions, masses).potential(None, theta, float(softening), G)
+
+
 class Tree:
     """Barnes-Hut tree object (api.py:432-786): build, refit, rebuild and
     accelerations, with the tree resident in device memory."""
@@ -178,6 +195,22 @@ class Tree:
             raise ValueError(f"theta must be non-negative, got {theta}")
         return barnes_hut_accelerations_targets(self._tree, self.positions, self.masses, targets, softening, theta,
                                                 G, precision=precision)
+
+    def potential(self, targets: Optional[ArrayLike] = None, *, theta: float = 0.5, softening: float = 0.0,
+                  G: float = 1.0) -> np.ndarray:
+        """Softened potential at the particles (self excluded) or at targets."""
+        if not self._is_built:
+            raise ValueError("Tree must be built before evaluating accelerations")
+        if not isinstance(softening, (int, float)) or softening < 0:
+            raise ValueError("Softening must be a non-negative scalar")
+        if theta < 0:
+            raise ValueError(f"theta must be non-negative, got {theta}")
+        if targets is not None:
+            targets = np.asarray(targets, dtype=np.float64)
+            if len(targets.shape) != 2 or targets.shape[1] != self.dim:
+                raise ValueError(f"targets must be (M, {self.dim}), got shape {targets.shape}")
+        return device_tree_for(self._tree, self.positions, self.masses).potential(targets, theta, float(softening),
+                                                                                  G)
 
     def evaluate_accelerations(self, softening: Union[float, ArrayLike], theta: float = 0.5, G: float = 1.0, *,
                                precision: str = "fp64") -> np.ndarray:

@@ -132,7 +132,7 @@ static void free_tree_arrays(bh_tree* t) {
   bh_ctx* c = t->ctx;
   void* ps[] = {t->spart, t->perm,  t->start, t->count, t->first_child, t->parent, t->skip,
                 t->leaf_order, t->center, t->half, t->mass, t->com, t->bmin, t->bmax, t->gbox,
-                t->nodes, t->spart32, t->pleaf, t->quad, t->d_sys_poff, t->d_sys_noff, t->d_gtab};
+                t->nodes, t->spart32, t->pleaf, t->quad, t->quad32, t->d_sys_poff, t->d_sys_noff, t->d_gtab};
   for (void* p : ps) dfree(c, p);
 }
 
@@ -829,6 +829,36 @@ int bh_accel_targets(bh_tree* t, const double* tgt, int64_t m, double theta, dou
   BH_HIP(hipMemcpyAsync(dt, tgt, (size_t)m * t->dim * 8, hipMemcpyHostToDevice, c->stream));
   BH_TRY(bh_accel_targets_dev(t, dt, m, theta, eps, G, d));
   BH_HIP(hipMemcpyAsync(out, d, (size_t)m * t->dim * 8, hipMemcpyDeviceToHost, c->stream));
+  BH_HIP(hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+// Softened potential through the plain walk: same node visits and opening
+// decisions as the accelerations, -G m / sqrt(r^2 + eps^2) per accepted node
+// and per leaf pair (self excluded in self mode).
+int bh_potential_dev(bh_tree* t, const double* d_tgt, int64_t m, double theta, double eps, double G,
+                     double* d_out) {
+  BH_REQ(t && (m == 0 || d_out), "NULL argument");
+  BH_REQ(d_tgt || m == t->n, "self mode needs m == n");
+  BH_REQ(!d_tgt || t->nsys == 1, "external targets need a single tree, not a forest");
+  BH_REQ(m >= 0 && m < 0x7fffffffll, "bad target count");
+  DeviceGuard g(t->ctx->device);
+  return walk_mode(t->ctx, t, WALK_POT, d_tgt, m, theta, eps, G, d_out, nullptr, nullptr, nullptr);
+}
+
+int bh_potential(bh_tree* t, const double* tgt, int64_t m, double theta, double eps, double G, double* out) {
+  BH_REQ(t && (m == 0 || out), "NULL argument");
+  BH_REQ(tgt || m == t->n, "self mode needs m == n");
+  if (m == 0) return BH_OK;
+  bh_ctx* c = t->ctx;
+  DeviceGuard g(c->device);
+  Scratch s(c);
+  double* dt = tgt ? s.get<double>((size_t)m * t->dim) : nullptr;
+  double* d = s.get<double>((size_t)m);
+  if ((tgt && !dt) || !d) return BH_ENOMEM;
+  if (tgt) BH_HIP(hipMemcpyAsync(dt, tgt, (size_t)m * t->dim * 8, hipMemcpyHostToDevice, c->stream));
+  BH_TRY(bh_potential_dev(t, dt, m, theta, eps, G, d));
+  BH_HIP(hipMemcpyAsync(out, d, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream));
   BH_HIP(hipStreamSynchronize(c->stream));
   return BH_OK;
 }

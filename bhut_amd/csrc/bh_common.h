@@ -156,6 +156,7 @@ struct bh_tree {
   float4* spart32 = nullptr;
   int32_t* pleaf = nullptr;
   double* quad = nullptr;  // M*6: xx, yy, zz, xy, xz, yz
+  float4* quad32 = nullptr;  // M*2: (xx, yy, zz, trace), (xy, xz, yz, 0) in FP32
   bool f32_ready = false;
   bool quad_ready = false;
   // forest group table of the FP32 form for group size gtab_gs: ng + 1
@@ -220,7 +221,7 @@ int pack_nodes(bh_ctx* c, bh_tree* t);
 int refit_check(bh_ctx* c, const bh_tree* t, const double* pos, int64_t n, int* ok);
 int fill_u32(bh_ctx* c, uint32_t* p, uint32_t v, int64_t n);
 // bh_walk.hip
-enum { WALK_ACC = 0, WALK_LCOUNT = 1, WALK_LFILL = 2, WALK_STATS = 3 };
+enum { WALK_ACC = 0, WALK_LCOUNT = 1, WALK_LFILL = 2, WALK_STATS = 3, WALK_POT = 4 };
 int order_targets(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, uint32_t* torder);
 int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t m, double theta,
               double eps, double G, double* out, const int64_t* lptr, int32_t* lids, int64_t* stats);

@@ -6,17 +6,19 @@
 //
 //  1. k_group_box: FP64 bounding box of each group's targets.
 //  2. k_group_walk: one wavefront per group walks the tree breadth-wise with
-//     an LDS stack: 64 nodes are tested per step, one per lane, with a
+//     an LDS stack: 128 nodes are tested per step, two per lane, with a
 //     conservative group test -- accept a node only if size < theta * dmin,
 //     dmin the distance from its centre of mass to the group box, so every
 //     target of the group would also accept it under bhut's own test
 //     (bh.py:586-601).  Lanes are compacted with ballots into the cell list,
 //     the leaf-chunk list and the stack.  Run twice: count, then fill.
-//  3. k_group_eval: one wavefront per group, one (or two) targets per lane.
+//  3. k_group_eval: persistent wavefronts over work items (a group's list
+//     cut into pieces of at most 64 tiles), one (or two) targets per lane.
 //     Tiles of 64 list entries are loaded cooperatively (coalesced) into one
 //     LDS tile per wavefront and read back as wave-wide broadcasts; the next
 //     tile's HBM reads are in flight in registers while the current tile is
-//     consumed, and groups run longest first.  All interactions are
+//     consumed.  k_item_reduce adds each target's item partials in item
+//     order.  All interactions are
 //     FP32 on coordinates relative to the group centre: sources are stored
 //     relative to their leaf centre and shifted by the FP64-formed
 //     leaf-to-group offset, so precision follows the leaf size, not the
@@ -127,11 +129,22 @@ __global__ __launch_bounds__(256) void k_quad_up(int64_t nleaves, const int32_t*
   }
 }
 
+// Accepted-cell payload of every node in the layout the cell list uses:
+// (Qxx, Qyy, Qzz, trace) and (Qxy, Qxz, Qyz, 0) in FP32.
+__global__ __launch_bounds__(256) void k_quad32(int64_t M, const double* __restrict__ quad, float4* __restrict__ q32) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  const double* q = quad + k * 6;
+  q32[2 * k] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)(q[0] + q[1] + q[2]));
+  q32[2 * k + 1] = make_float4((float)q[3], (float)q[4], (float)q[5], 0.f);
+}
+
 int f32_prepare(bh_ctx* c, bh_tree* t) {
   if (!t->spart32) t->spart32 = dalloc_t<float4>(c, (size_t)t->n);
   if (!t->pleaf) t->pleaf = dalloc_t<int32_t>(c, (size_t)t->n);
   if (!t->quad) t->quad = dalloc_t<double>(c, (size_t)t->M * 6);
-  if (!t->spart32 || !t->pleaf || !t->quad) return BH_ENOMEM;
+  if (!t->quad32) t->quad32 = dalloc_t<float4>(c, (size_t)t->M * 2);
+  if (!t->spart32 || !t->pleaf || !t->quad || !t->quad32) return BH_ENOMEM;
   const unsigned nb = (unsigned)cdiv(t->nleaves, 256);
   if (t->dim == 3)
     hipLaunchKernelGGL(k_src32<3>, dim3(nb), dim3(256), 0, c->stream, t->nleaves, t->leaf_order, t->start,
@@ -148,6 +161,7 @@ int f32_prepare(bh_ctx* c, bh_tree* t) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_quad_up, dim3(nb), dim3(256), 0, c->stream, t->nleaves, t->leaf_order, t->start,
                        t->count, t->first_child, t->parent, t->spart, t->com, t->dim, t->mass, t->quad, flags);
+    hipLaunchKernelGGL(k_quad32, dim3((unsigned)cdiv(t->M, 256)), dim3(256), 0, c->stream, t->M, t->quad, t->quad32);
     BH_LAUNCH_CHECK();
     t->quad_ready = true;
   } else {
@@ -217,10 +231,15 @@ __global__ void k_gather_targets(const double* __restrict__ tgt, const uint32_t*
 }
 
 // ---------------------------------------------------------------- walk
-template <int DIM, bool FILL>
+constexpr int WALK_W = 2;  // node records per lane per step
+
+// Each step pops up to 64*W stack entries, W per lane, so W node records
+// per lane are in flight at once (the walk is bound by their latency).  The
+// output order (step, j, lane) is the same in the count and fill passes.
+template <int DIM, bool FILL, int W>
 __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nodes,
                                                    const double* __restrict__ center,
-                                                   const double* __restrict__ quad, int use_quad,
+                                                   const float4* __restrict__ quad32, int use_quad,
                                                    const GroupBox* __restrict__ boxes, double theta2,
                                                    int stack_cap, int ch, int64_t* __restrict__ ccount,
                                                    int64_t* __restrict__ kcount, int64_t* __restrict__ pcount,
@@ -243,99 +262,103 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
   const int64_t kb = FILL ? koff[g] : 0, ke = FILL ? koff[g + 1] : 0;
   const uint64_t lt = lanemask_lt();
   while (sp > 0) {
-    const int k = sp < 64 ? sp : 64;
-    int v = -1;
-    if (lane < k) v = stack[sp - k + lane];
+    const int k = sp < 64 * W ? sp : 64 * W;
+    int v[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) v[j] = lane + 64 * j < k ? stack[sp - k + lane + 64 * j] : -1;
     __syncthreads();
     sp -= k;
-    int kind = 0;  // 1 cell, 2 leaf, 3 open
-    int nchunk = 0;
-    NodeD nd;
-    if (v >= 0) {
-      nd = nodes[v];
-      if (nd.m != 0.0) {
-        if (nd.fc < 0) {
+    NodeD nd[W];
+#pragma unroll
+    for (int j = 0; j < W; j++)
+      if (v[j] >= 0) nd[j] = nodes[v[j]];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      int kind = 0;  // 1 cell, 2 leaf, 3 open
+      int nchunk = 0;
+      if (v[j] >= 0 && nd[j].m != 0.0) {
+        if (nd[j].fc < 0) {
           kind = 2;
-          nchunk = (nd.count + ch - 1) / ch;
+          nchunk = (nd[j].count + ch - 1) / ch;
         } else {
-          double ex = fabs(nd.cx - B.cx) - B.hx;
-          double ey = fabs(nd.cy - B.cy) - B.hy;
-          double ez = DIM == 3 ? fabs(nd.cz - B.cz) - B.hz : 0.0;
+          double ex = fabs(nd[j].cx - B.cx) - B.hx;
+          double ey = fabs(nd[j].cy - B.cy) - B.hy;
+          double ez = DIM == 3 ? fabs(nd[j].cz - B.cz) - B.hz : 0.0;
           ex = ex > 0.0 ? ex : 0.0;
           ey = ey > 0.0 ? ey : 0.0;
           ez = ez > 0.0 ? ez : 0.0;
           const double d2 = ex * ex + ey * ey + ez * ez;
-          const bool open = !(d2 > 0.0) || !(nd.size * nd.size < theta2 * d2);
+          const bool open = !(d2 > 0.0) || !(nd[j].size * nd[j].size < theta2 * d2);
           kind = open ? 3 : 1;
         }
       }
-    }
-    const uint64_t bc = __ballot(kind == 1), bo = __ballot(kind == 3);
-    // exclusive prefix of chunk counts over the lanes (wave scan)
-    int kx = nchunk;
+      const uint64_t bc = __ballot(kind == 1), bo = __ballot(kind == 3);
+      // exclusive prefix of chunk counts over the lanes (wave scan)
+      int kx = nchunk;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(kx, off);
-      if (lane >= off) kx += y;
-    }
-    const int ktot = __shfl(kx, 63);
-    kx -= nchunk;
-    if (kind == 3) {
-      const int o = sp + 2 * __popcll(bo & lt);
-      if (o + 1 < stack_cap) {
-        stack[o] = nd.fc;
-        stack[o + 1] = nd.fc + 1;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(kx, off);
+        if (lane >= off) kx += y;
       }
-    }
-    if (FILL) {
-      if (kind == 1) {
-        const int64_t o = cb + nc + __popcll(bc & lt);
-        if (o < ce) {
-          CellEnt e;
-          e.a = make_float4((float)(nd.cx - B.cx), (float)(nd.cy - B.cy), (float)(nd.cz - B.cz), (float)nd.m);
-          if (use_quad) {
-            const double* q = quad + (int64_t)v * 6;
-            e.b = make_float4((float)q[0], (float)q[1], (float)q[2], (float)(q[0] + q[1] + q[2]));
-            e.c = make_float4((float)q[3], (float)q[4], (float)q[5], 0.f);
-          } else {
-            e.b = make_float4(0.f, 0.f, 0.f, 0.f);
-            e.c = make_float4(0.f, 0.f, 0.f, 0.f);
-          }
-          clist[o] = e;
+      const int ktot = __shfl(kx, 63);
+      kx -= nchunk;
+      if (kind == 3) {
+        const int o = sp + 2 * __popcll(bo & lt);
+        if (o + 1 < stack_cap) {
+          stack[o] = nd[j].fc;
+          stack[o + 1] = nd[j].fc + 1;
         }
       }
-      if (kind == 2) {
-        const float sx = (float)(center[(int64_t)v * DIM] - B.cx);
-        const float sy = (float)(center[(int64_t)v * DIM + 1] - B.cy);
-        const float sz = DIM == 3 ? (float)(center[(int64_t)v * DIM + 2] - B.cz) : 0.f;
-        for (int q = 0; q < nchunk; q++) {
-          const int64_t o = kb + nk + kx + q;
-          if (o < ke) {
-            ChunkEnt e;
-            e.sx = sx;
-            e.sy = sy;
-            e.sz = sz;
-            e.start = nd.start + q * ch;
-            e.count = min(ch, nd.count - q * ch);
-            e.pad[0] = e.pad[1] = e.pad[2] = 0;
-            klist[o] = e;
+      if (FILL) {
+        if (kind == 1) {
+          const int64_t o = cb + nc + __popcll(bc & lt);
+          if (o < ce) {
+            CellEnt e;
+            e.a = make_float4((float)(nd[j].cx - B.cx), (float)(nd[j].cy - B.cy), (float)(nd[j].cz - B.cz),
+                              (float)nd[j].m);
+            if (use_quad) {
+              e.b = quad32[2 * (int64_t)v[j]];
+              e.c = quad32[2 * (int64_t)v[j] + 1];
+            } else {
+              e.b = make_float4(0.f, 0.f, 0.f, 0.f);
+              e.c = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            clist[o] = e;
           }
         }
-      }
-    } else {
-      int pc = kind == 2 ? nd.count : 0;
+        if (kind == 2) {
+          const float sx = (float)(center[(int64_t)v[j] * DIM] - B.cx);
+          const float sy = (float)(center[(int64_t)v[j] * DIM + 1] - B.cy);
+          const float sz = DIM == 3 ? (float)(center[(int64_t)v[j] * DIM + 2] - B.cz) : 0.f;
+          for (int q = 0; q < nchunk; q++) {
+            const int64_t o = kb + nk + kx + q;
+            if (o < ke) {
+              ChunkEnt e;
+              e.sx = sx;
+              e.sy = sy;
+              e.sz = sz;
+              e.start = nd[j].start + q * ch;
+              e.count = min(ch, nd[j].count - q * ch);
+              e.pad[0] = e.pad[1] = e.pad[2] = 0;
+              klist[o] = e;
+            }
+          }
+        }
+      } else {
+        int pc = kind == 2 ? nd[j].count : 0;
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
-      np += pc;
+        for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
+        np += pc;
+      }
+      nc += __popcll(bc);
+      nk += ktot;
+      const int push = 2 * __popcll(bo);
+      if (sp + push > stack_cap) {
+        if (lane == 0) atomicOr(err, 1u);
+        return;
+      }
+      sp += push;
     }
-    nc += __popcll(bc);
-    nk += ktot;
-    const int push = 2 * __popcll(bo);
-    if (sp + push > stack_cap) {
-      if (lane == 0) atomicOr(err, 1u);
-      return;
-    }
-    sp += push;
     __syncthreads();
   }
   if (!FILL && lane == 0) {
@@ -881,12 +904,12 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   {
     ProfScope p(c, "k_group_walk_count");
     if (t->dim == 3)
-      hipLaunchKernelGGL((k_group_walk<3, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
-                         t->quad, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
+      hipLaunchKernelGGL((k_group_walk<3, false, WALK_W>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
+                         t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
                          nullptr, nullptr, nullptr, err, groot, (int64_t)0);
     else
-      hipLaunchKernelGGL((k_group_walk<2, false>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
-                         t->quad, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
+      hipLaunchKernelGGL((k_group_walk<2, false, WALK_W>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
+                         t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
                          nullptr, nullptr, nullptr, err, groot, (int64_t)0);
     BH_LAUNCH_CHECK();
   }
@@ -994,12 +1017,12 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     {
       ProfScope p(c, "k_group_walk_fill");
       if (t->dim == 3)
-        hipLaunchKernelGGL((k_group_walk<3, true>), dim3((unsigned)nb), dim3(64), lds, c->stream, t->nodes,
-                           t->center, t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
+        hipLaunchKernelGGL((k_group_walk<3, true, WALK_W>), dim3((unsigned)nb), dim3(64), lds, c->stream, t->nodes,
+                           t->center, t->quad32, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
                            offs, offs + ng + 1, clist, klist, err, groot, b.g0);
       else
-        hipLaunchKernelGGL((k_group_walk<2, true>), dim3((unsigned)nb), dim3(64), lds, c->stream, t->nodes,
-                           t->center, t->quad, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
+        hipLaunchKernelGGL((k_group_walk<2, true, WALK_W>), dim3((unsigned)nb), dim3(64), lds, c->stream, t->nodes,
+                           t->center, t->quad32, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
                            offs, offs + ng + 1, clist, klist, err, groot, b.g0);
       BH_LAUNCH_CHECK();
     }

@@ -4,7 +4,9 @@
 // _barnes_hut_single_target_external (bh.py:83-146), _should_open_node
 // (bh.py:576-601), acc_monopole_node/acc_monopole (bhut/traverse/kernels.py:
 // 14-69,136-168), the Numba leaf loops (bh.py:174-280) with their pure-Python
-// fallback (bh.py:307-311,339-343,384-425).
+// fallback (bh.py:307-311,339-343,384-425).  WALK_POT evaluates the softened
+// potential over the same visits (bhut documents `bhut.potential`,
+// docs/getting-started.md:36, without implementing it).
 //
 // One lane per target, targets in Morton order so a wavefront walks similar
 // paths.  The reference's LIFO stack pops the right child first, i.e. visits
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
     ty = tgt[orig * DIM + 1];
     if (DIM == 3) tz = tgt[orig * DIM + 2];
   }
-  double ax = 0.0, ay = 0.0, az = 0.0;
+  double ax = 0.0, ay = 0.0, az = 0.0, phi = 0.0;
   int64_t nl = 0, wpos = 0;
   int64_t c_pop = 0, c_test = 0, c_mono = 0, c_leaf = 0, c_pair = 0;
   if (MODE == WALK_LFILL) wpos = lptr[orig];
@@ -118,6 +120,22 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
         ay = ay + py;
         if (DIM == 3) az = az + pz;
       }
+      if (MODE == WALK_POT) {
+        // softened pair potential -G m / sqrt(r^2 + eps^2), zero where the
+        // softened distance is zero; leaf partial added to the total
+        double pp = 0.0;
+        for (int64_t j = nd.start; j < nd.start + nd.count; j++) {
+          if (SELF && j == t) continue;
+          const double4 q = spart[j];
+          const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+          double r2 = dx * dx;
+          r2 = r2 + dy * dy;
+          if (DIM == 3) r2 = r2 + dz * dz;
+          const double r2s = r2 + eps2;
+          if (r2s != 0.0) pp = pp - (G * q.w) / sqrt(r2s);
+        }
+        phi = phi + pp;
+      }
       v = nd.skip;
       continue;
     }
@@ -138,6 +156,14 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
     if (MODE == WALK_LCOUNT) nl++;
     if (MODE == WALK_LFILL) lids[wpos++] = v;
     if (MODE == WALK_STATS) c_mono++;
+    if (MODE == WALK_POT) {
+      const double dx = nd.cx - tx, dy = nd.cy - ty, dz = nd.cz - tz;
+      double r2 = dx * dx;
+      r2 = r2 + dy * dy;
+      if (DIM == 3) r2 = r2 + dz * dz;
+      const double r2s = r2 + eps2;
+      if (r2s != 0.0) phi = phi - (G * nd.m) / sqrt(r2s);
+    }
     if (MODE == WALK_ACC) {
       const double dx = nd.cx - tx, dy = nd.cy - ty, dz = nd.cz - tz;
       double r2 = dx * dx;
@@ -162,6 +188,7 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
     out[orig * DIM + 1] = ay;
     if (DIM == 3) out[orig * DIM + 2] = az;
   }
+  if (MODE == WALK_POT) out[orig] = phi;
   if (MODE == WALK_LCOUNT) stats[orig] = nl;
   if (MODE == WALK_STATS) {
     int64_t* o = stats + orig * 5;
@@ -186,6 +213,7 @@ static void launch_walk(bh_ctx* c, int mode, const bh_tree* t, int64_t m, const 
     case WALK_ACC: BH_W(WALK_ACC); break;
     case WALK_LCOUNT: BH_W(WALK_LCOUNT); break;
     case WALK_LFILL: BH_W(WALK_LFILL); break;
+    case WALK_POT: BH_W(WALK_POT); break;
     default: BH_W(WALK_STATS); break;
   }
 #undef BH_W
@@ -225,8 +253,9 @@ int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t 
     int rc = order_targets(c, t, tgt, m, torder);
     if (rc) return rc;
   }
-  static const char* names[] = {"k_walk_f64", "k_walk_f64_lcount", "k_walk_f64_lfill", "k_walk_f64_stats"};
-  ProfScope ps(c, names[mode & 3]);
+  static const char* names[] = {"k_walk_f64", "k_walk_f64_lcount", "k_walk_f64_lfill", "k_walk_f64_stats",
+                                "k_walk_f64_pot"};
+  ProfScope ps(c, names[mode]);
   if (t->dim == 3) {
     if (tgt) launch_walk<3, false>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
     else launch_walk<3, true>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);

@@ -145,6 +145,19 @@ int bh_accel_f32_targets_dev(bh_tree *t, const double *d_targets, int64_t m, dou
                              double softening, double G, const bh_f32_opts *opts, double *d_out,
                              int64_t *stats);
 
+/* ------------------------------------------------------------ potential */
+/* Softened gravitational potential phi_i = -G sum m_j / sqrt(r^2 + eps^2)
+ * through the plain FP64 walk (the same visits and opening decisions as
+ * bh_accel_self / bh_accel_targets), out has m entries in the original
+ * target order.  bhut documents `bhut.potential(positions, masses, theta)`
+ * (docs/getting-started.md:36) but does not implement it; there is no
+ * reference function to follow.  targets == NULL selects self mode
+ * (m == n, self pair excluded). */
+int bh_potential(bh_tree *t, const double *targets, int64_t m, double theta, double softening, double G,
+                 double *out);
+int bh_potential_dev(bh_tree *t, const double *d_targets, int64_t m, double theta, double softening,
+                     double G, double *d_out);
+
 /* ------------------------------------------------- lists and statistics */
 /* Per-target interaction lists of the walk in _barnes_hut_single_target
  * (bh.py:485-548): node ids in pop order, leaves and accepted monopoles.

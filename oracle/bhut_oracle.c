@@ -483,6 +483,34 @@ static int leaf_numba(const orc_tree *t, int64_t leaf, const double *tp, int64_t
   return 1;
 }
 
+/* Softened potential of one source, G m / sqrt(r2 + eps^2) with r2 summed
+ * x, y, z in order (0 when the softened distance is 0).  bhut has no
+ * potential; this is the definition the library's walk evaluates. */
+static double pair_potential(const double *dx, double m, double eps, double G, int dim) {
+  double r2 = dx[0] * dx[0];
+  for (int d = 1; d < dim; d++) { double q = dx[d] * dx[d]; r2 = r2 + q; }
+  double r2s = r2 + eps * eps;
+  if (r2s == 0.0) return 0.0;
+  return (G * m) / sqrt(r2s);
+}
+
+/* Potential of one leaf: partial sum from 0.0 in leaf order, self skipped. */
+static double leaf_potential(const orc_tree *t, int64_t leaf, const double *tp, int64_t tidx,
+                             const double *pos, const double *mass, double eps, double G) {
+  const int dim = (int)t->dim;
+  double pp = 0.0;
+  int64_t s = t->start[leaf], c = t->count[leaf];
+  if (s == -1 || c <= 0) return 0.0;
+  for (int64_t i = s; i < s + c; i++) {
+    int64_t p = t->perm[i];
+    if (p == tidx) continue;
+    double dx[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < dim; d++) dx[d] = pos[p * dim + d] - tp[d];
+    pp = pp - pair_potential(dx, mass[p], eps, G, dim);
+  }
+  return pp;
+}
+
 /* _should_open_node (bh.py:576-601): d = np.linalg.norm(target - com);
  * d == 0 always opens; otherwise open iff (2*half_size)/d >= theta. */
 static int should_open_theta(const orc_tree *t, int64_t k, const double *tp, double theta) {
@@ -501,6 +529,7 @@ static int should_open_theta(const orc_tree *t, int64_t k, const double *tp, dou
  * The stack holds at most depth+1 <= 130 entries. */
 typedef struct {
   double *acc;        /* accumulate here (NULL: do not compute) */
+  double *phi;        /* softened potential (NULL: do not compute) */
   int64_t *list;      /* interaction list (NULL: not recorded) */
   int64_t list_cap, list_len;
   int64_t cnt[5];     /* pops, tests, monopoles, leaves, pairs */
@@ -527,6 +556,7 @@ static void walk(const orc_tree *t, const double *tp, int64_t tidx, const double
           if (t->perm[i] == tidx) pairs--;
       w->cnt[4] += pairs;
       if (w->list) { if (w->list_len < w->list_cap) w->list[w->list_len] = k; w->list_len++; }
+      if (w->phi) *w->phi = *w->phi + leaf_potential(t, k, tp, tidx, pos, mass, eps, G);
       if (w->acc) {
         double a[3];
         if (leaf_formula == ORC_LEAF_PYTHON || !leaf_numba(t, k, tp, tidx, pos, mass, eps, G, a))
@@ -543,6 +573,11 @@ static void walk(const orc_tree *t, const double *tp, int64_t tidx, const double
       } else {
         w->cnt[2]++;
         if (w->list) { if (w->list_len < w->list_cap) w->list[w->list_len] = k; w->list_len++; }
+        if (w->phi) {
+          double dx[3] = {0.0, 0.0, 0.0};
+          for (int d = 0; d < dim; d++) dx[d] = t->com[k * dim + d] - tp[d];
+          *w->phi = *w->phi - pair_potential(dx, t->mass[k], eps, G, dim);
+        }
         if (w->acc) {
           double dx[3], a[3];
           for (int d = 0; d < dim; d++) dx[d] = t->com[k * dim + d] - tp[d];
@@ -575,6 +610,23 @@ void orc_accel(const orc_tree *t, const double *src_pos, const double *src_mass,
     memset(&w, 0, sizeof(w));
     w.acc = out + i * dim;
     walk(t, tgt + i * dim, self_mode ? i : -1, src_pos, src_mass, eps, theta, G, leaf_formula, &w);
+  }
+}
+
+/* Potential over the same walk as orc_accel: accepted nodes and leaf partial
+ * sums added in pop order. */
+void orc_potential(const orc_tree *t, const double *src_pos, const double *src_mass,
+                   const double *tgt, int64_t m, int self_mode, double eps, double theta,
+                   double G, int nthreads, double *out) {
+  const int dim = (int)t->dim;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < m; i++) {
+    walk_out w;
+    memset(&w, 0, sizeof(w));
+    out[i] = 0.0;
+    w.phi = out + i;
+    walk(t, tgt + i * dim, self_mode ? i : -1, src_pos, src_mass, eps, theta, G, ORC_LEAF_NUMBA, &w);
   }
 }
 

@@ -71,6 +71,13 @@ void orc_accel(const orc_tree *t, const double *src_pos, const double *src_mass,
                double theta, double G, int leaf_formula, int nthreads,
                double *out);
 
+/* Softened potential -G sum m / sqrt(r2 + eps2) over the same walk
+ * (accepted nodes and leaf partial sums in pop order, self skipped in
+ * self_mode).  No bhut counterpart: the definition the library follows. */
+void orc_potential(const orc_tree *t, const double *src_pos, const double *src_mass,
+                   const double *tgt, int64_t m, int self_mode, double eps, double theta,
+                   double G, int nthreads, double *out);
+
 /* Interaction list of one target in pop order: node ids of leaves summed
  * directly and internal nodes used as monopoles.  Returns the list length
  * (entries beyond cap are counted but not written). */

@@ -74,6 +74,7 @@ def lib():
         L.orc_should_refit.restype = ci
         L.orc_should_refit.argtypes = [ctypes.POINTER(_OrcTree), vp, i64, ci]
         L.orc_accel.argtypes = [ctypes.POINTER(_OrcTree), vp, vp, vp, i64, ci, dbl, dbl, dbl, ci, ci, vp]
+        L.orc_potential.argtypes = [ctypes.POINTER(_OrcTree), vp, vp, vp, i64, ci, dbl, dbl, dbl, ci, vp]
         L.orc_target_list.restype = i64
         L.orc_target_list.argtypes = [ctypes.POINTER(_OrcTree), vp, dbl, vp, i64]
         L.orc_counters.argtypes = [ctypes.POINTER(_OrcTree), vp, i64, ci, dbl, ci, vp]
@@ -221,6 +222,19 @@ def accelerations(tree: OracleTree, src_pos, src_mass, targets=None, *, softenin
     st = tree._struct()
     lib().orc_accel(ctypes.byref(st), _p(src_pos), _p(src_mass), _p(tgt), len(tgt), int(self_mode),
                     float(softening), float(theta), float(G), int(leaf_formula), int(nthreads), _p(out))
+    return out
+
+
+def potential(tree: OracleTree, src_pos, src_mass, targets=None, *, softening=0.0, theta=0.5, G=1.0,
+              nthreads=0):
+    """Softened potential over the walk (no bhut counterpart; see orc_potential)."""
+    src_pos, src_mass = _f64(src_pos), _f64(src_mass)
+    self_mode = targets is None
+    tgt = src_pos if self_mode else _f64(targets)
+    out = np.zeros(len(tgt))
+    st = tree._struct()
+    lib().orc_potential(ctypes.byref(st), _p(src_pos), _p(src_mass), _p(tgt), len(tgt), int(self_mode),
+                        float(softening), float(theta), float(G), int(nthreads), _p(out))
     return out
 
 
