@@ -231,13 +231,23 @@ __global__ void k_gather_targets(const double* __restrict__ tgt, const uint32_t*
 }
 
 // ---------------------------------------------------------------- walk
-constexpr int WALK_W = 2;  // node records per lane per step
+// Each wavefront touches only LDS it owns (its stack, its tile), and a
+// wavefront's LDS operations complete in issue order, so a compiler-level
+// wave barrier orders its own writes and reads; no workgroup barrier.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int WALK_W = 2;      // node records per lane per step
+constexpr int WALK_WAVES = 4;  // groups (wavefronts) per workgroup, each with its own LDS stack
 
 // Each step pops up to 64*W stack entries, W per lane, so W node records
 // per lane are in flight at once (the walk is bound by their latency).  The
 // output order (step, j, lane) is the same in the count and fill passes.
 template <int DIM, bool FILL, int W>
-__global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nodes,
+__global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __restrict__ nodes,
                                                    const double* __restrict__ center,
                                                    const float4* __restrict__ quad32, int use_quad,
                                                    const GroupBox* __restrict__ boxes, double theta2,
@@ -247,15 +257,19 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
                                                    const int64_t* __restrict__ koff,
                                                    CellEnt* __restrict__ clist, ChunkEnt* __restrict__ klist,
                                                    uint32_t* __restrict__ err, const int32_t* __restrict__ groot,
-                                                   int64_t g0) {
-  extern __shared__ int32_t stack[];
+                                                   int64_t g0, int64_t nlaunch, int wpb) {
+  extern __shared__ int32_t lds_stacks[];
   // lists of fixed capacity that would overflow are not written at all
   if (FILL && (__builtin_amdgcn_readfirstlane(*err) & 2u)) return;
-  const int64_t g = g0 + blockIdx.x;
-  const int lane = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int64_t gl = (int64_t)blockIdx.x * wpb + wv;
+  if (wv >= wpb || gl >= nlaunch) return;  // whole wavefront
+  const int64_t g = g0 + gl;
+  const int lane = threadIdx.x % 64;
+  int32_t* stack = lds_stacks + wv * stack_cap;
   const GroupBox B = boxes[g];
   if (lane == 0) stack[0] = groot ? groot[g] : 0;  // the group's own system
-  __syncthreads();
+  wave_sync();
   int sp = 1;
   int64_t nc = 0, nk = 0, np = 0;
   const int64_t cb = FILL ? coff[g] : 0, ce = FILL ? coff[g + 1] : 0;
@@ -266,7 +280,7 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
     int v[W];
 #pragma unroll
     for (int j = 0; j < W; j++) v[j] = lane + 64 * j < k ? stack[sp - k + lane + 64 * j] : -1;
-    __syncthreads();
+    wave_sync();
     sp -= k;
     NodeD nd[W];
 #pragma unroll
@@ -359,7 +373,7 @@ __global__ __launch_bounds__(64) void k_group_walk(const NodeD* __restrict__ nod
       }
       sp += push;
     }
-    __syncthreads();
+    wave_sync();
   }
   if (!FILL && lane == 0) {
     ccount[g] = nc;
@@ -440,14 +454,6 @@ __device__ __forceinline__ void flush(float (&px)[TPL], float (&py)[TPL], float 
   }
 }
 
-// Each wavefront reads only the LDS tile it wrote itself, and a wavefront's
-// LDS operations complete in issue order, so a compiler-level wave barrier
-// orders the tile hand-off; no workgroup barrier is needed.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 constexpr int EV_WAVES = 4;  // groups (wavefronts) per workgroup
 
@@ -900,17 +906,25 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   }
   hipLaunchKernelGGL(k_group_box, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, gt0, boxes);
   const double theta2 = theta * theta;
-  const size_t lds = (size_t)stack_cap * sizeof(int32_t);
+  // wavefronts per workgroup: WALK_WAVES unless their stacks exceed the
+  // workgroup's LDS limit
+  int max_lds = 65536;
+  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess) {
+    (void)hipGetLastError();
+    max_lds = 65536;
+  }
+  const int wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / ((int64_t)stack_cap * 4)));
+  const size_t lds = (size_t)wpb * stack_cap * sizeof(int32_t);
   {
     ProfScope p(c, "k_group_walk_count");
     if (t->dim == 3)
-      hipLaunchKernelGGL((k_group_walk<3, false, WALK_W>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
+      hipLaunchKernelGGL((k_group_walk<3, false, WALK_W>), dim3((unsigned)cdiv(ng, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes, t->center,
                          t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err, groot, (int64_t)0);
+                         nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb);
     else
-      hipLaunchKernelGGL((k_group_walk<2, false, WALK_W>), dim3((unsigned)ng), dim3(64), lds, c->stream, t->nodes, t->center,
+      hipLaunchKernelGGL((k_group_walk<2, false, WALK_W>), dim3((unsigned)cdiv(ng, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes, t->center,
                          t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err, groot, (int64_t)0);
+                         nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb);
     BH_LAUNCH_CHECK();
   }
   // Lists are materialised for batches of consecutive groups whose lists
@@ -1017,13 +1031,13 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     {
       ProfScope p(c, "k_group_walk_fill");
       if (t->dim == 3)
-        hipLaunchKernelGGL((k_group_walk<3, true, WALK_W>), dim3((unsigned)nb), dim3(64), lds, c->stream, t->nodes,
+        hipLaunchKernelGGL((k_group_walk<3, true, WALK_W>), dim3((unsigned)cdiv(nb, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,
                            t->center, t->quad32, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
-                           offs, offs + ng + 1, clist, klist, err, groot, b.g0);
+                           offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb);
       else
-        hipLaunchKernelGGL((k_group_walk<2, true, WALK_W>), dim3((unsigned)nb), dim3(64), lds, c->stream, t->nodes,
+        hipLaunchKernelGGL((k_group_walk<2, true, WALK_W>), dim3((unsigned)cdiv(nb, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,
                            t->center, t->quad32, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
-                           offs, offs + ng + 1, clist, klist, err, groot, b.g0);
+                           offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb);
       BH_LAUNCH_CHECK();
     }
     {
