@@ -594,7 +594,7 @@ __device__ __forceinline__ void eval_item(
       src_cur = load_src(idx_cur);
       load_chunk(e + 2 * CPT, ka_nxt, kb_nxt);
       const int nslots = (int)((ke - e < CPT ? ke - e : CPT) * CH);
-#pragma unroll 4
+#pragma unroll 8
       for (int k = 0; k < nslots; k++) {
         const float4 v = tile[0][k];
 #pragma unroll
