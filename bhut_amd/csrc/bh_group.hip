@@ -222,6 +222,48 @@ __global__ __launch_bounds__(64) void k_group_box(const double4* __restrict__ tp
   }
 }
 
+// Eight sub-boxes per group, of gs/8 consecutive targets each (empty
+// sub-boxes have lo = +inf, hi = -inf): the exact group test below only
+// measures targets in the sub-boxes a node's opening ball can reach.
+constexpr int NSUB = 8;
+struct SubBox {
+  double4 lo, hi;
+};
+
+__global__ __launch_bounds__(64) void k_group_sub(const double4* __restrict__ tpos, int64_t m, int gs,
+                                                  const int32_t* __restrict__ gt0, SubBox* __restrict__ sub) {
+  const int64_t g = blockIdx.x;
+  const int lane = threadIdx.x;
+  int64_t t0, t1;
+  group_range(g, m, gs, gt0, t0, t1);
+  const int per = gs / NSUB;  // targets per sub-box: 8 or 16
+  for (int h = 0; h < gs / 64; h++) {
+    const int64_t i = t0 + 64 * h + lane;
+    double lx = __builtin_inf(), ly = __builtin_inf(), lz = __builtin_inf();
+    double ux = -__builtin_inf(), uy = -__builtin_inf(), uz = -__builtin_inf();
+    if (i < t1) {
+      const double4 p = tpos[i];
+      lx = ux = p.x;
+      ly = uy = p.y;
+      lz = uz = p.z;
+    }
+    for (int off = per / 2; off > 0; off >>= 1) {
+      lx = fmin(lx, __shfl_xor(lx, off));
+      ly = fmin(ly, __shfl_xor(ly, off));
+      lz = fmin(lz, __shfl_xor(lz, off));
+      ux = fmax(ux, __shfl_xor(ux, off));
+      uy = fmax(uy, __shfl_xor(uy, off));
+      uz = fmax(uz, __shfl_xor(uz, off));
+    }
+    if (lane % per == 0) {
+      SubBox b;
+      b.lo = make_double4(lx, ly, lz, 0.0);
+      b.hi = make_double4(ux, uy, uz, 0.0);
+      sub[g * NSUB + (64 * h + lane) / per] = b;
+    }
+  }
+}
+
 __global__ void k_gather_targets(const double* __restrict__ tgt, const uint32_t* __restrict__ order, int64_t m,
                                  int dim, double4* __restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -257,7 +299,10 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
                                                    const int64_t* __restrict__ koff,
                                                    CellEnt* __restrict__ clist, ChunkEnt* __restrict__ klist,
                                                    uint32_t* __restrict__ err, const int32_t* __restrict__ groot,
-                                                   int64_t g0, int64_t nlaunch, int wpb) {
+                                                   int64_t g0, int64_t nlaunch, int wpb,
+                                                   const double4* __restrict__ tpos, int64_t m, int gs,
+                                                   const int32_t* __restrict__ gt0,
+                                                   const SubBox* __restrict__ subs) {
   extern __shared__ int32_t lds_stacks[];
   // lists of fixed capacity that would overflow are not written at all
   if (FILL && (__builtin_amdgcn_readfirstlane(*err) & 2u)) return;
@@ -267,6 +312,16 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
   const int64_t g = g0 + gl;
   const int lane = threadIdx.x % 64;
   int32_t* stack = lds_stacks + wv * stack_cap;
+  // the group's sub-boxes and targets in LDS, for the exact test below
+  SubBox* sb = reinterpret_cast<SubBox*>(lds_stacks + wpb * stack_cap) +
+               (int64_t)wv * (NSUB + gs / 2);  // per wavefront: NSUB sub-boxes, then gs targets
+  double4* tg = reinterpret_cast<double4*>(sb + NSUB);
+  int64_t t0, t1;
+  group_range(g, m, gs, gt0, t0, t1);
+  const int ntg = (int)(t1 - t0), per = gs / NSUB;
+  if (lane < NSUB) sb[lane] = subs[g * NSUB + lane];
+  for (int q = lane; q < ntg; q += 64) tg[q] = tpos[t0 + q];
+  const double4 T0 = tpos[t0];
   const GroupBox B = boxes[g];
   if (lane == 0) stack[0] = groot ? groot[g] : 0;  // the group's own system
   wave_sync();
@@ -302,8 +357,44 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
           ey = ey > 0.0 ? ey : 0.0;
           ez = ez > 0.0 ? ez : 0.0;
           const double d2 = ex * ex + ey * ey + ez * ez;
-          const bool open = !(d2 > 0.0) || !(nd[j].size * nd[j].size < theta2 * d2);
+          const double s2 = nd[j].size * nd[j].size;
+          const bool open = !(d2 > 0.0) || !(s2 < theta2 * d2);
           kind = open ? 3 : 1;
+          if (open) {
+            // the box test opens; the exact test -- every target of the
+            // group accepts, i.e. size^2 < theta^2 min_t |t - com|^2 -- can
+            // still accept when the box is loose around its targets.  Its
+            // necessary condition uses the first target.
+            const double ux = T0.x - nd[j].cx, uy = T0.y - nd[j].cy, uz = DIM == 3 ? T0.z - nd[j].cz : 0.0;
+            if (s2 < theta2 * (ux * ux + uy * uy + uz * uz)) kind = 4;
+          }
+        }
+      }
+      // exact test: accept iff no target lies within size/theta of the
+      // centre of mass; only sub-boxes the opening ball reaches are searched
+      if (__ballot(kind == 4)) {
+        if (kind == 4) {
+          const double s2 = nd[j].size * nd[j].size;
+          const double cx = nd[j].cx, cy = nd[j].cy, cz = DIM == 3 ? nd[j].cz : 0.0;
+          bool near = false;
+          for (int k = 0; k < NSUB && !near; k++) {
+            const double4 lo = sb[k].lo, hi = sb[k].hi;
+            double ex = fmax(fmax(lo.x - cx, cx - hi.x), 0.0);
+            double ey = fmax(fmax(lo.y - cy, cy - hi.y), 0.0);
+            double ez = DIM == 3 ? fmax(fmax(lo.z - cz, cz - hi.z), 0.0) : 0.0;
+            if (!(s2 < theta2 * (ex * ex + ey * ey + ez * ez))) {
+              const int qe = (k + 1) * per < ntg ? (k + 1) * per : ntg;
+              for (int q = k * per; q < qe; q++) {
+                const double4 tq = tg[q];
+                const double ux = tq.x - cx, uy = tq.y - cy, uz = DIM == 3 ? tq.z - cz : 0.0;
+                if (!(s2 < theta2 * (ux * ux + uy * uy + uz * uz))) {
+                  near = true;
+                  break;
+                }
+              }
+            }
+          }
+          kind = near ? 3 : 1;
         }
       }
       const uint64_t bc = __ballot(kind == 1), bo = __ballot(kind == 3);
@@ -825,7 +916,9 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   if (m == 0) return BH_OK;
   const int gs = o && o->group_size ? o->group_size : 64;
   const bool kahan = o ? o->kahan != 0 : true;
-  const int stack_cap = o && o->stack_cap ? o->stack_cap : 2048;
+  // LDS stack entries per wavefront, a multiple of 16 so what follows the
+  // stacks in LDS stays 64-byte aligned
+  const int stack_cap = ((o && o->stack_cap ? o->stack_cap : 2048) + 15) & ~15;
   const int multipole = o && o->multipole ? o->multipole : 2;
   if (gs != 64 && gs != 128) {
     set_error("group_size must be 64 or 128, got %d", gs);
@@ -905,6 +998,9 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_group_box, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, gt0, boxes);
+  SubBox* subs = s.get<SubBox>((size_t)ng * NSUB);
+  if (!subs) return BH_ENOMEM;
+  hipLaunchKernelGGL(k_group_sub, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, gt0, subs);
   const double theta2 = theta * theta;
   // wavefronts per workgroup: WALK_WAVES unless their stacks exceed the
   // workgroup's LDS limit
@@ -913,18 +1009,19 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     (void)hipGetLastError();
     max_lds = 65536;
   }
-  const int wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / ((int64_t)stack_cap * 4)));
-  const size_t lds = (size_t)wpb * stack_cap * sizeof(int32_t);
+  const int64_t per_wave = (int64_t)stack_cap * 4 + (int64_t)(NSUB + gs / 2) * (int64_t)sizeof(SubBox);
+  const int wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / per_wave));
+  const size_t lds = (size_t)wpb * per_wave;
   {
     ProfScope p(c, "k_group_walk_count");
     if (t->dim == 3)
       hipLaunchKernelGGL((k_group_walk<3, false, WALK_W>), dim3((unsigned)cdiv(ng, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes, t->center,
                          t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb);
+                         nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb, tpos, m, gs, gt0, subs);
     else
       hipLaunchKernelGGL((k_group_walk<2, false, WALK_W>), dim3((unsigned)cdiv(ng, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes, t->center,
                          t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb);
+                         nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb, tpos, m, gs, gt0, subs);
     BH_LAUNCH_CHECK();
   }
   // Lists are materialised for batches of consecutive groups whose lists
@@ -1033,11 +1130,11 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       if (t->dim == 3)
         hipLaunchKernelGGL((k_group_walk<3, true, WALK_W>), dim3((unsigned)cdiv(nb, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,
                            t->center, t->quad32, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
-                           offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb);
+                           offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos, m, gs, gt0, subs);
       else
         hipLaunchKernelGGL((k_group_walk<2, true, WALK_W>), dim3((unsigned)cdiv(nb, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,
                            t->center, t->quad32, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
-                           offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb);
+                           offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos, m, gs, gt0, subs);
       BH_LAUNCH_CHECK();
     }
     {
