@@ -95,6 +95,9 @@ struct bh_ctx {
   bool capturing = false;
   // bytes of FP32 interaction lists materialised at once (0: default)
   size_t list_budget = 0;
+  // largest group-walk stack an earlier call had to grow to; later calls
+  // start there (the walk's lists do not depend on the stack size)
+  int stack_hint = 0;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   struct ProfRec {

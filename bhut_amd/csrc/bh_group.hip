@@ -283,6 +283,7 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 constexpr int WALK_W = 2;      // node records per lane per step
+constexpr int MAX_STACK = 8192;  // LDS stack entries per wavefront, at most
 constexpr int WALK_WAVES = 4;  // groups (wavefronts) per workgroup, each with its own LDS stack
 
 // Each step pops up to 64*W stack entries, W per lane, so W node records
@@ -918,20 +919,21 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const bool kahan = o ? o->kahan != 0 : true;
   // LDS stack entries per wavefront, a multiple of 16 so what follows the
   // stacks in LDS stays 64-byte aligned
-  const int stack_cap = ((o && o->stack_cap ? o->stack_cap : 2048) + 15) & ~15;
+  int stack_cap = ((o && o->stack_cap ? o->stack_cap : 2048) + 15) & ~15;
   const int multipole = o && o->multipole ? o->multipole : 2;
   if (gs != 64 && gs != 128) {
     set_error("group_size must be 64 or 128, got %d", gs);
     return BH_EINVAL;
   }
-  if (stack_cap < 64 || stack_cap > 8192) {
-    set_error("stack_cap must be in [64, 8192], got %d", stack_cap);
+  if (stack_cap < 64 || stack_cap > MAX_STACK) {
+    set_error("stack_cap must be in [64, %d], got %d", MAX_STACK, stack_cap);
     return BH_EINVAL;
   }
   if (multipole != 1 && multipole != 2) {
     set_error("multipole must be 1 (monopole) or 2 (quadrupole), got %d", multipole);
     return BH_EINVAL;
   }
+  if (!plan && c->stack_hint > stack_cap) stack_cap = c->stack_hint;
   if (!t->f32_ready) {
     ProfScope p(c, "f32_prepare");
     int rc = f32_prepare(c, t);
@@ -1009,10 +1011,14 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     (void)hipGetLastError();
     max_lds = 65536;
   }
-  const int64_t per_wave = (int64_t)stack_cap * 4 + (int64_t)(NSUB + gs / 2) * (int64_t)sizeof(SubBox);
-  const int wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / per_wave));
-  const size_t lds = (size_t)wpb * per_wave;
-  {
+  int wpb = 1;
+  size_t lds = 0;
+  // the count pass; an LDS stack overflow on the host-scanned path reruns it
+  // with twice the stack (the lists do not depend on the stack size)
+  auto count_pass = [&]() -> int {
+    const int64_t per_wave = (int64_t)stack_cap * 4 + (int64_t)(NSUB + gs / 2) * (int64_t)sizeof(SubBox);
+    wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / per_wave));
+    lds = (size_t)wpb * per_wave;
     ProfScope p(c, "k_group_walk_count");
     if (t->dim == 3)
       hipLaunchKernelGGL((k_group_walk<3, false, WALK_W>), dim3((unsigned)cdiv(ng, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes, t->center,
@@ -1023,6 +1029,11 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
                          t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
                          nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb, tpos, m, gs, gt0, subs);
     BH_LAUNCH_CHECK();
+    return BH_OK;
+  };
+  {
+    int rc = count_pass();
+    if (rc) return rc;
   }
   // Lists are materialised for batches of consecutive groups whose lists
   // fit the list budget (one batch unless the problem is very large); each
@@ -1050,13 +1061,21 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     batches.push_back({0, ng, max_items});
   } else {
     hc.resize(3 * ng);
-    uint32_t herr = 0;
-    BH_HIP(hipMemcpyAsync(hc.data(), cnt, 3 * ng * 8, hipMemcpyDeviceToHost, c->stream));
-    BH_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
-    BH_HIP(hipStreamSynchronize(c->stream));
-    if (herr) {
-      set_error("group walk stack overflow (stack_cap=%d); raise bh_f32_opts.stack_cap", stack_cap);
-      return BH_EOVERFLOW;
+    for (;;) {
+      uint32_t herr = 0;
+      BH_HIP(hipMemcpyAsync(hc.data(), cnt, 3 * ng * 8, hipMemcpyDeviceToHost, c->stream));
+      BH_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+      BH_HIP(hipStreamSynchronize(c->stream));
+      if (!herr) break;
+      if (stack_cap >= MAX_STACK) {
+        set_error("group walk stack overflow (stack_cap=%d)", stack_cap);
+        return BH_EOVERFLOW;
+      }
+      stack_cap = std::min(MAX_STACK, 2 * stack_cap);
+      c->stack_hint = std::max(c->stack_hint, stack_cap);
+      int rc = fill_u32(c, err, 0u, 1);
+      if (!rc) rc = count_pass();
+      if (rc) return rc;
     }
     const size_t budget = c->list_budget ? c->list_budget : LIST_BUDGET;
     ho.assign(2 * (ng + 1), 0);

@@ -134,8 +134,9 @@ int capture(bh_sim* s, int kind, double dt) {
   return BH_OK;
 }
 
-// Reads the status word; on a list overflow grows the capacities, redoes
-// the force and the final kick eagerly and drops the captured graphs.
+// Reads the status word; on a list or walk-stack overflow grows the
+// capacity, redoes the force and the final kick eagerly and drops the
+// captured graphs.
 int settle(bh_sim* s, double dt) {
   bh_ctx* c = s->ctx;
   for (int attempt = 0;; attempt++) {
@@ -144,17 +145,22 @@ int settle(bh_sim* s, double dt) {
     BH_HIP(hipStreamSynchronize(c->stream));
     const uint32_t st = (uint32_t)h[0];
     if (st == 0) return BH_OK;
+    if (attempt >= 8) {
+      set_error("interaction lists or the walk stack still overflow after regrowing");
+      return BH_EOVERFLOW;
+    }
     if (st & 1u) {
-      set_error("group walk stack overflow (stack_cap=%d); raise bh_f32_opts.stack_cap", s->opts.stack_cap);
-      return BH_EOVERFLOW;
+      // the LDS stack of the group walk: twice as many entries
+      if (s->opts.stack_cap >= 8192) {
+        set_error("group walk stack overflow (stack_cap=%d)", s->opts.stack_cap);
+        return BH_EOVERFLOW;
+      }
+      s->opts.stack_cap = std::min(8192, 2 * s->opts.stack_cap);
+    } else {
+      const int64_t nc = (int64_t)h[1], nk = (int64_t)h[2];
+      s->cap_cells = std::max(s->cap_cells, nc + nc / 4 + 4096);
+      s->cap_chunks = std::max(s->cap_chunks, nk + nk / 4 + 4096);
     }
-    if (attempt >= 2) {
-      set_error("interaction lists still overflow after regrowing");
-      return BH_EOVERFLOW;
-    }
-    const int64_t nc = (int64_t)h[1], nk = (int64_t)h[2];
-    s->cap_cells = std::max(s->cap_cells, nc + nc / 4 + 4096);
-    s->cap_chunks = std::max(s->cap_chunks, nk + nk / 4 + 4096);
     s->regrows++;
     drop_graph(s, 0);
     drop_graph(s, 1);
