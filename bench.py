@@ -208,7 +208,7 @@ def workload_plummer(args, env):
         probe = nat.DeviceTree.build_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, leaf, context=env.ctx)
         best, table = autotune.tune_f32(probe, theta, eps, G, multipole=args.multipole, kahan=not args.no_kahan,
                                         group_sizes=(args.group_size,) if args.group_size else (64, 128),
-                                        stack_caps=(args.stack_cap,) if args.stack_cap else (1024, 2048),
+                                        stack_caps=(args.stack_cap,) if args.stack_cap else (512, 1024),
                                         out_ptr=out_t.data_ptr())
         del probe
         tuning = {"chosen": best, "table": table}
@@ -341,7 +341,7 @@ def workload_ensemble(args, env):
     pos_t, mass_t = env.upload(x), env.upload(m)
     out_t = torch.empty((N, 3), dtype=torch.float64, device=env.dev)
     env.sync()
-    opts = f32_options(args, {"group_size": args.group_size or 64, "stack_cap": args.stack_cap or 2048})
+    opts = f32_options(args, {"group_size": args.group_size or 64, "stack_cap": args.stack_cap or 1024})
 
     def forest_step(form):
         f = nat.DeviceTree.build_forest_dev(pos_t.data_ptr(), mass_t.data_ptr(), counts, 3, leaf, context=env.ctx)
@@ -421,7 +421,7 @@ def workload_diskhalo(args, env):
     for theta in (0.3, 0.5, 0.7):
         log(f"bench: theta={theta}: autotuning leaf / group size / stack depth")
         best, table = autotune.tune_leaf_f32_dev(pos_t.data_ptr(), mass_t.data_ptr(), N, 3, out_t.data_ptr(),
-                                                 theta, eps, G, stack_caps=(2048,), multipole=args.multipole,
+                                                 theta, eps, G, stack_caps=(1024,), multipole=args.multipole,
                                                  context=env.ctx)
         opts = f32_options(args, best)
         leaf = best["leaf_size"]

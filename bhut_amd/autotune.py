@@ -30,7 +30,7 @@ def _timed(ctx, fn, reps: int) -> float:
 
 
 def tune_f32(tree: "_native.DeviceTree", theta: float, softening: float, G: float = 1.0, *, targets=None,
-             group_sizes: Iterable[int] = (64, 128), stack_caps: Iterable[int] = (1024, 2048), multipole: int = 2,
+             group_sizes: Iterable[int] = (64, 128), stack_caps: Iterable[int] = (512, 1024), multipole: int = 2,
              kahan: bool = True, reps: int = 2, out_ptr: int | None = None):
     """Best (group_size, stack_cap) for this tree: returns (best, table).
     With out_ptr (device buffer of n x dim float64) self accelerations stay
@@ -61,7 +61,7 @@ def tune_f32(tree: "_native.DeviceTree", theta: float, softening: float, G: floa
 
 def tune_leaf_f32(positions, masses, theta: float, softening: float, G: float = 1.0, *,
                   leaf_sizes: Iterable[int] = (16, 32, 64), group_sizes: Iterable[int] = (64, 128),
-                  stack_caps: Iterable[int] = (2048,), multipole: int = 2, reps: int = 1, context=None):
+                  stack_caps: Iterable[int] = (1024,), multipole: int = 2, reps: int = 1, context=None):
     """Best (leaf_size, group_size, stack_cap) including the tree build."""
     ctx = context or _native.ctx()
     pos = np.ascontiguousarray(positions, dtype=np.float64)
@@ -86,7 +86,7 @@ def tune_leaf_f32(positions, masses, theta: float, softening: float, G: float = 
 
 def tune_leaf_f32_dev(pos_ptr: int, mass_ptr: int, n: int, dim: int, out_ptr: int, theta: float,
                       softening: float, G: float = 1.0, *, leaf_sizes: Iterable[int] = (16, 32, 64),
-                      group_sizes: Iterable[int] = (64, 128), stack_caps: Iterable[int] = (2048,),
+                      group_sizes: Iterable[int] = (64, 128), stack_caps: Iterable[int] = (1024,),
                       multipole: int = 2, reps: int = 1, context=None):
     """tune_leaf_f32 on inputs already in device memory."""
     ctx = context or _native.ctx()

@@ -95,9 +95,6 @@ struct bh_ctx {
   bool capturing = false;
   // bytes of FP32 interaction lists materialised at once (0: default)
   size_t list_budget = 0;
-  // largest group-walk stack an earlier call had to grow to; later calls
-  // start there (the walk's lists do not depend on the stack size)
-  int stack_hint = 0;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   struct ProfRec {
@@ -168,6 +165,10 @@ struct bh_tree {
   int gtab_gs = 0;
   int64_t gtab_ng = 0;
   std::vector<int32_t> h_gtab;
+  // largest group-walk stack an earlier call on this tree had to grow to,
+  // for self (0) and external (1) targets; later calls start there (the
+  // walk's lists do not depend on the stack size)
+  int stack_hint[2] = {0, 0};
 };
 
 namespace bh {

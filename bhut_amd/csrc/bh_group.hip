@@ -282,7 +282,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int WALK_W = 2;      // node records per lane per step
+constexpr int WALK_W = 1;      // node records per lane per step
 constexpr int MAX_STACK = 8192;  // LDS stack entries per wavefront, at most
 constexpr int WALK_WAVES = 4;  // groups (wavefronts) per workgroup, each with its own LDS stack
 
@@ -919,7 +919,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const bool kahan = o ? o->kahan != 0 : true;
   // LDS stack entries per wavefront, a multiple of 16 so what follows the
   // stacks in LDS stays 64-byte aligned
-  int stack_cap = ((o && o->stack_cap ? o->stack_cap : 2048) + 15) & ~15;
+  int stack_cap = ((o && o->stack_cap ? o->stack_cap : 1024) + 15) & ~15;
   const int multipole = o && o->multipole ? o->multipole : 2;
   if (gs != 64 && gs != 128) {
     set_error("group_size must be 64 or 128, got %d", gs);
@@ -933,7 +933,8 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     set_error("multipole must be 1 (monopole) or 2 (quadrupole), got %d", multipole);
     return BH_EINVAL;
   }
-  if (!plan && c->stack_hint > stack_cap) stack_cap = c->stack_hint;
+  int& hint = t->stack_hint[tgt ? 1 : 0];
+  if (!plan && hint > stack_cap) stack_cap = hint;
   if (!t->f32_ready) {
     ProfScope p(c, "f32_prepare");
     int rc = f32_prepare(c, t);
@@ -1072,7 +1073,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
         return BH_EOVERFLOW;
       }
       stack_cap = std::min(MAX_STACK, 2 * stack_cap);
-      c->stack_hint = std::max(c->stack_hint, stack_cap);
+      hint = std::max(hint, stack_cap);
       int rc = fill_u32(c, err, 0u, 1);
       if (!rc) rc = count_pass();
       if (rc) return rc;

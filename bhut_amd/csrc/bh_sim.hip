@@ -213,7 +213,7 @@ int bh_sim_create(bh_ctx* c, double* d_pos, double* d_vel, const double* d_mass,
   if (opts) s->opts = *opts;
   else s->opts.kahan = 1;
   if (!s->opts.group_size) s->opts.group_size = 64;
-  if (!s->opts.stack_cap) s->opts.stack_cap = 2048;
+  if (!s->opts.stack_cap) s->opts.stack_cap = 1024;
   if (!s->opts.multipole) s->opts.multipole = 2;
   int rc = bh_tree_build_dev(c, d_pos, d_mass, n, dim, leaf_size, &s->tree);
   if (rc == BH_OK && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)

@@ -40,7 +40,7 @@ typedef struct {
   int32_t group_size; /* targets per group: 64 (one per lane) or 128 [64]   */
   int32_t kahan;      /* 1: Kahan-compensated per-target sums, 0: plain     */
   int32_t stack_cap;  /* initial LDS stack entries of the group walk, <= 8192;
-                         doubled (up to 8192) when a walk overflows [2048] */
+                         doubled (up to 8192) when a walk overflows [1024] */
   int32_t multipole;  /* accepted cells: 1 monopole, 2 quadrupole [2]       */
 } bh_f32_opts;
 
