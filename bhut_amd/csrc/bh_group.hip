@@ -6,12 +6,15 @@
 //
 //  1. k_group_box: FP64 bounding box of each group's targets.
 //  2. k_group_walk: one wavefront per group walks the tree breadth-wise with
-//     an LDS stack: 128 nodes are tested per step, two per lane, with a
-//     conservative group test -- accept a node only if size < theta * dmin,
-//     dmin the distance from its centre of mass to the group box, so every
-//     target of the group would also accept it under bhut's own test
-//     (bh.py:586-601).  Lanes are compacted with ballots into the cell list,
-//     the leaf-chunk list and the stack.  Run twice: count, then fill.
+//     an LDS stack: up to 64 nodes are tested per step, one per lane (WALK_W),
+//     with a conservative group test -- accept a node if size < theta * dmin,
+//     dmin the distance from its centre of mass to the group box, or, when
+//     that box test opens, if no target of the group lies within size/theta
+//     of the centre of mass (exact test, searched through eight sub-boxes of
+//     the group).  Either way every target of the group would also accept
+//     the node under bhut's own test (bh.py:586-601).  Lanes are compacted
+//     with ballots into the cell list, the leaf-chunk list and the stack.
+//     Run twice: count, then fill.
 //  3. k_group_eval: persistent wavefronts over work items (a group's list
 //     cut into pieces of at most 64 tiles), one (or two) targets per lane.
 //     Tiles of 64 list entries are loaded cooperatively (coalesced) into one
