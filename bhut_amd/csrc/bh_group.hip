@@ -6,13 +6,13 @@
 //
 //  1. k_group_box: FP64 bounding box of each group's targets.
 //  2. k_group_walk: one wavefront per group walks the tree breadth-wise with
-//     an LDS stack: up to 64 nodes are tested per step, one per lane (WALK_W),
-//     with a conservative group test -- accept a node if size < theta * dmin,
-//     dmin the distance from its centre of mass to the group box, or, when
-//     that box test opens, if no target of the group lies within size/theta
-//     of the centre of mass (exact test, searched through eight sub-boxes of
-//     the group).  Either way every target of the group would also accept
-//     the node under bhut's own test (bh.py:586-601).  Lanes are compacted
+//     an LDS stack: up to 64 nodes are tested per step, one per lane, with a
+//     conservative group test -- accept a node if size < theta * dmin, dmin
+//     the distance from its centre of mass to the group box, or, when that
+//     box test opens, if no target of the group lies within size/theta of the
+//     centre of mass (exact test: the lanes hold the targets and measure one
+//     candidate node at a time).  Either way every target of the group would
+//     also accept the node under bhut's own test (bh.py:586-601).  Lanes are compacted
 //     with ballots into the cell list, the leaf-chunk list and the stack.
 //     Run twice: count, then fill.
 //  3. k_group_eval: persistent wavefronts over work items (a group's list
@@ -23,7 +23,7 @@
 //     consumed.  k_item_reduce adds each target's item partials in item
 //     order.  All interactions are
 //     FP32 on coordinates relative to the group centre: sources are stored
-//     relative to their leaf centre and shifted by the FP64-formed
+//     relative to their leaf's centre of mass and shifted by the FP64-formed
 //     leaf-to-group offset, so precision follows the leaf size, not the
 //     coordinate size.  Each tile's partial sum is added to the per-target
 //     total with Kahan compensation.
@@ -49,7 +49,7 @@ struct __attribute__((aligned(16))) CellEnt {
 
 // A run of at most CH particles of one leaf.
 struct __attribute__((aligned(16))) ChunkEnt {
-  float sx, sy, sz;  // leaf centre - group centre
+  float sx, sy, sz;  // leaf centre of mass - group centre
   int32_t start;     // first particle (sorted order)
   int32_t count;     // particles in this chunk
   int32_t pad[3];
@@ -57,18 +57,20 @@ struct __attribute__((aligned(16))) ChunkEnt {
 static_assert(sizeof(ChunkEnt) == 32, "ChunkEnt layout");
 
 // ---------------------------------------------------------------- prepare
-// Sources relative to their leaf's centre (FP32) and each particle's leaf.
+// Sources relative to their leaf's centre of mass (FP32) and each
+// particle's leaf.  The centre of mass is the one the node record carries, so
+// the walk forms the leaf-to-group shift from the record it already holds.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_src32(int64_t nleaves, const int32_t* __restrict__ leaf_order,
                                                const int32_t* __restrict__ start,
                                                const int32_t* __restrict__ count,
-                                               const double* __restrict__ center,
+                                               const double* __restrict__ com,
                                                const double4* __restrict__ spart,
                                                float4* __restrict__ src32, int32_t* __restrict__ pleaf) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nleaves) return;
   const int v = leaf_order[t];
-  const double cx = center[v * DIM], cy = center[v * DIM + 1], cz = DIM == 3 ? center[v * DIM + 2] : 0.0;
+  const double cx = com[v * DIM], cy = com[v * DIM + 1], cz = DIM == 3 ? com[v * DIM + 2] : 0.0;
   const int64_t s = start[v], c = count[v];
   for (int64_t j = s; j < s + c; j++) {
     const double4 p = spart[j];
@@ -151,10 +153,10 @@ int f32_prepare(bh_ctx* c, bh_tree* t) {
   const unsigned nb = (unsigned)cdiv(t->nleaves, 256);
   if (t->dim == 3)
     hipLaunchKernelGGL(k_src32<3>, dim3(nb), dim3(256), 0, c->stream, t->nleaves, t->leaf_order, t->start,
-                       t->count, t->center, t->spart, t->spart32, t->pleaf);
+                       t->count, t->com, t->spart, t->spart32, t->pleaf);
   else
     hipLaunchKernelGGL(k_src32<2>, dim3(nb), dim3(256), 0, c->stream, t->nleaves, t->leaf_order, t->start,
-                       t->count, t->center, t->spart, t->spart32, t->pleaf);
+                       t->count, t->com, t->spart, t->spart32, t->pleaf);
   BH_LAUNCH_CHECK();
   if (t->binary) {
     Scratch s(c);
@@ -225,48 +227,6 @@ __global__ __launch_bounds__(64) void k_group_box(const double4* __restrict__ tp
   }
 }
 
-// Eight sub-boxes per group, of gs/8 consecutive targets each (empty
-// sub-boxes have lo = +inf, hi = -inf): the exact group test below only
-// measures targets in the sub-boxes a node's opening ball can reach.
-constexpr int NSUB = 8;
-struct SubBox {
-  double4 lo, hi;
-};
-
-__global__ __launch_bounds__(64) void k_group_sub(const double4* __restrict__ tpos, int64_t m, int gs,
-                                                  const int32_t* __restrict__ gt0, SubBox* __restrict__ sub) {
-  const int64_t g = blockIdx.x;
-  const int lane = threadIdx.x;
-  int64_t t0, t1;
-  group_range(g, m, gs, gt0, t0, t1);
-  const int per = gs / NSUB;  // targets per sub-box: 8 or 16
-  for (int h = 0; h < gs / 64; h++) {
-    const int64_t i = t0 + 64 * h + lane;
-    double lx = __builtin_inf(), ly = __builtin_inf(), lz = __builtin_inf();
-    double ux = -__builtin_inf(), uy = -__builtin_inf(), uz = -__builtin_inf();
-    if (i < t1) {
-      const double4 p = tpos[i];
-      lx = ux = p.x;
-      ly = uy = p.y;
-      lz = uz = p.z;
-    }
-    for (int off = per / 2; off > 0; off >>= 1) {
-      lx = fmin(lx, __shfl_xor(lx, off));
-      ly = fmin(ly, __shfl_xor(ly, off));
-      lz = fmin(lz, __shfl_xor(lz, off));
-      ux = fmax(ux, __shfl_xor(ux, off));
-      uy = fmax(uy, __shfl_xor(uy, off));
-      uz = fmax(uz, __shfl_xor(uz, off));
-    }
-    if (lane % per == 0) {
-      SubBox b;
-      b.lo = make_double4(lx, ly, lz, 0.0);
-      b.hi = make_double4(ux, uy, uz, 0.0);
-      sub[g * NSUB + (64 * h + lane) / per] = b;
-    }
-  }
-}
-
 __global__ void k_gather_targets(const double* __restrict__ tgt, const uint32_t* __restrict__ order, int64_t m,
                                  int dim, double4* __restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -280,21 +240,29 @@ __global__ void k_gather_targets(const double* __restrict__ tgt, const uint32_t*
 // wavefront's LDS operations complete in issue order, so a compiler-level
 // wave barrier orders its own writes and reads; no workgroup barrier.
 __device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-constexpr int WALK_W = 1;      // node records per lane per step
 constexpr int MAX_STACK = 8192;  // LDS stack entries per wavefront, at most
-constexpr int WALK_WAVES = 4;  // groups (wavefronts) per workgroup, each with its own LDS stack
+constexpr int WALK_WAVES = 4;    // groups (wavefronts) per workgroup, each with its own LDS stack
 
-// Each step pops up to 64*W stack entries, W per lane, so W node records
-// per lane are in flight at once (the walk is bound by their latency).  The
-// output order (step, j, lane) is the same in the count and fill passes.
-template <int DIM, bool FILL, int W>
+// The double held by lane j of the wavefront (j uniform).
+__device__ __forceinline__ double readlane_d(double v, int j) {
+  const int64_t b = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)b, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double((int64_t)(((uint64_t)hi << 32) | lo));
+}
+
+// Each step pops up to 64 stack entries, one per lane, and tests those nodes
+// at once; the output order (step, lane) is the same in the count and fill
+// passes.  TPW = group size / 64 targets per lane: the exact test runs
+// across the lanes, every lane measuring its own targets against one
+// candidate node at a time.
+template <int DIM, bool FILL, int TPW>
 __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __restrict__ nodes,
-                                                   const double* __restrict__ center,
                                                    const float4* __restrict__ quad32, int use_quad,
                                                    const GroupBox* __restrict__ boxes, double theta2,
                                                    int stack_cap, int ch, int64_t* __restrict__ ccount,
@@ -304,9 +272,8 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
                                                    CellEnt* __restrict__ clist, ChunkEnt* __restrict__ klist,
                                                    uint32_t* __restrict__ err, const int32_t* __restrict__ groot,
                                                    int64_t g0, int64_t nlaunch, int wpb,
-                                                   const double4* __restrict__ tpos, int64_t m, int gs,
-                                                   const int32_t* __restrict__ gt0,
-                                                   const SubBox* __restrict__ subs) {
+                                                   const double4* __restrict__ tpos, int64_t m,
+                                                   const int32_t* __restrict__ gt0) {
   extern __shared__ int32_t lds_stacks[];
   // lists of fixed capacity that would overflow are not written at all
   if (FILL && (__builtin_amdgcn_readfirstlane(*err) & 2u)) return;
@@ -316,15 +283,16 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
   const int64_t g = g0 + gl;
   const int lane = threadIdx.x % 64;
   int32_t* stack = lds_stacks + wv * stack_cap;
-  // the group's sub-boxes and targets in LDS, for the exact test below
-  SubBox* sb = reinterpret_cast<SubBox*>(lds_stacks + wpb * stack_cap) +
-               (int64_t)wv * (NSUB + gs / 2);  // per wavefront: NSUB sub-boxes, then gs targets
-  double4* tg = reinterpret_cast<double4*>(sb + NSUB);
   int64_t t0, t1;
-  group_range(g, m, gs, gt0, t0, t1);
-  const int ntg = (int)(t1 - t0), per = gs / NSUB;
-  if (lane < NSUB) sb[lane] = subs[g * NSUB + lane];
-  for (int q = lane; q < ntg; q += 64) tg[q] = tpos[t0 + q];
+  group_range(g, m, 64 * TPW, gt0, t0, t1);
+  // this lane's targets; lanes past the group's end repeat its first target,
+  // which is measured anyway
+  double4 tq[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int64_t i = t0 + lane + 64 * q;
+    tq[q] = tpos[i < t1 ? i : t0];
+  }
   const double4 T0 = tpos[t0];
   const GroupBox B = boxes[g];
   if (lane == 0) stack[0] = groot ? groot[g] : 0;  // the group's own system
@@ -335,146 +303,163 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
   const int64_t kb = FILL ? koff[g] : 0, ke = FILL ? koff[g + 1] : 0;
   const uint64_t lt = lanemask_lt();
   while (sp > 0) {
-    const int k = sp < 64 * W ? sp : 64 * W;
-    int v[W];
-#pragma unroll
-    for (int j = 0; j < W; j++) v[j] = lane + 64 * j < k ? stack[sp - k + lane + 64 * j] : -1;
+    const int k = sp < 64 ? sp : 64;
+    const int v = lane < k ? stack[sp - k + lane] : -1;
     wave_sync();
     sp -= k;
-    NodeD nd[W];
-#pragma unroll
-    for (int j = 0; j < W; j++)
-      if (v[j] >= 0) nd[j] = nodes[v[j]];
-#pragma unroll
-    for (int j = 0; j < W; j++) {
-      int kind = 0;  // 1 cell, 2 leaf, 3 open
-      int nchunk = 0;
-      if (v[j] >= 0 && nd[j].m != 0.0) {
-        if (nd[j].fc < 0) {
-          kind = 2;
-          nchunk = (nd[j].count + ch - 1) / ch;
-        } else {
-          double ex = fabs(nd[j].cx - B.cx) - B.hx;
-          double ey = fabs(nd[j].cy - B.cy) - B.hy;
-          double ez = DIM == 3 ? fabs(nd[j].cz - B.cz) - B.hz : 0.0;
-          ex = ex > 0.0 ? ex : 0.0;
-          ey = ey > 0.0 ? ey : 0.0;
-          ez = ez > 0.0 ? ez : 0.0;
-          const double d2 = ex * ex + ey * ey + ez * ez;
-          const double s2 = nd[j].size * nd[j].size;
-          const bool open = !(d2 > 0.0) || !(s2 < theta2 * d2);
-          kind = open ? 3 : 1;
-          if (open) {
-            // the box test opens; the exact test -- every target of the
-            // group accepts, i.e. size^2 < theta^2 min_t |t - com|^2 -- can
-            // still accept when the box is loose around its targets.  Its
-            // necessary condition uses the first target.
-            const double ux = T0.x - nd[j].cx, uy = T0.y - nd[j].cy, uz = DIM == 3 ? T0.z - nd[j].cz : 0.0;
-            if (s2 < theta2 * (ux * ux + uy * uy + uz * uz)) kind = 4;
-          }
+    NodeD nd;
+    if (v >= 0) nd = nodes[v];
+    int kind = 0;  // 1 cell, 2 leaf, 3 open
+    int nchunk = 0;
+    const double s2 = nd.size * nd.size;
+    if (v >= 0 && nd.m != 0.0) {
+      if (nd.fc < 0) {
+        kind = 2;
+        nchunk = (nd.count + ch - 1) / ch;
+      } else {
+        double ex = fabs(nd.cx - B.cx) - B.hx;
+        double ey = fabs(nd.cy - B.cy) - B.hy;
+        double ez = DIM == 3 ? fabs(nd.cz - B.cz) - B.hz : 0.0;
+        ex = ex > 0.0 ? ex : 0.0;
+        ey = ey > 0.0 ? ey : 0.0;
+        ez = ez > 0.0 ? ez : 0.0;
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        const bool open = !(d2 > 0.0) || !(s2 < theta2 * d2);
+        kind = open ? 3 : 1;
+        if (open) {
+          // the box test opens; the exact test -- every target of the
+          // group accepts, i.e. size^2 < theta^2 min_t |t - com|^2 -- can
+          // still accept when the box is loose around its targets.  Its
+          // necessary condition uses the first target.
+          const double ux = T0.x - nd.cx, uy = T0.y - nd.cy, uz = DIM == 3 ? T0.z - nd.cz : 0.0;
+          if (s2 < theta2 * (ux * ux + uy * uy + uz * uz)) kind = 4;
         }
       }
-      // exact test: accept iff no target lies within size/theta of the
-      // centre of mass; only sub-boxes the opening ball reaches are searched
-      if (__ballot(kind == 4)) {
-        if (kind == 4) {
-          const double s2 = nd[j].size * nd[j].size;
-          const double cx = nd[j].cx, cy = nd[j].cy, cz = DIM == 3 ? nd[j].cz : 0.0;
-          bool near = false;
-          for (int k = 0; k < NSUB && !near; k++) {
-            const double4 lo = sb[k].lo, hi = sb[k].hi;
-            double ex = fmax(fmax(lo.x - cx, cx - hi.x), 0.0);
-            double ey = fmax(fmax(lo.y - cy, cy - hi.y), 0.0);
-            double ez = DIM == 3 ? fmax(fmax(lo.z - cz, cz - hi.z), 0.0) : 0.0;
-            if (!(s2 < theta2 * (ex * ex + ey * ey + ez * ez))) {
-              const int qe = (k + 1) * per < ntg ? (k + 1) * per : ntg;
-              for (int q = k * per; q < qe; q++) {
-                const double4 tq = tg[q];
-                const double ux = tq.x - cx, uy = tq.y - cy, uz = DIM == 3 ? tq.z - cz : 0.0;
-                if (!(s2 < theta2 * (ux * ux + uy * uy + uz * uz))) {
-                  near = true;
-                  break;
-                }
-              }
-            }
-          }
-          kind = near ? 3 : 1;
+    }
+    // exact test, one candidate node at a time across the lanes: accept iff
+    // no target lies within size/theta of the centre of mass
+    uint64_t cand = __ballot(kind == 4);
+    if (cand) {
+      uint64_t nearm = 0;
+      do {
+        const int j = __builtin_ctzll(cand);
+        cand &= cand - 1;
+        const double cx = readlane_d(nd.cx, j), cy = readlane_d(nd.cy, j);
+        const double cz = DIM == 3 ? readlane_d(nd.cz, j) : 0.0;
+        const double sj = readlane_d(s2, j);
+        bool near = false;
+#pragma unroll
+        for (int q = 0; q < TPW; q++) {
+          const double ux = tq[q].x - cx, uy = tq[q].y - cy, uz = DIM == 3 ? tq[q].z - cz : 0.0;
+          near = near || !(sj < theta2 * (ux * ux + uy * uy + uz * uz));
         }
-      }
-      const uint64_t bc = __ballot(kind == 1), bo = __ballot(kind == 3);
-      // exclusive prefix of chunk counts over the lanes (wave scan)
-      int kx = nchunk;
+        if (__ballot(near)) nearm |= 1ull << j;
+      } while (cand);
+      if (kind == 4) kind = (nearm >> lane) & 1 ? 3 : 1;
+    }
+    const uint64_t bc = __ballot(kind == 1), bo = __ballot(kind == 3);
+    // exclusive prefix of chunk counts over the lanes: ballot counts when no
+    // leaf needs more than one chunk (leaf_size <= CH), else a wave scan
+    int kx, ktot;
+    if (__ballot(nchunk > 1) == 0) {
+      const uint64_t bk = __ballot(nchunk == 1);
+      kx = __popcll(bk & lt);
+      ktot = __popcll(bk);
+    } else {
+      kx = nchunk;
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
         const int y = __shfl_up(kx, off);
         if (lane >= off) kx += y;
       }
-      const int ktot = __shfl(kx, 63);
+      ktot = __shfl(kx, 63);
       kx -= nchunk;
-      if (kind == 3) {
-        const int o = sp + 2 * __popcll(bo & lt);
-        if (o + 1 < stack_cap) {
-          stack[o] = nd[j].fc;
-          stack[o + 1] = nd[j].fc + 1;
-        }
-      }
-      if (FILL) {
-        if (kind == 1) {
-          const int64_t o = cb + nc + __popcll(bc & lt);
-          if (o < ce) {
-            CellEnt e;
-            e.a = make_float4((float)(nd[j].cx - B.cx), (float)(nd[j].cy - B.cy), (float)(nd[j].cz - B.cz),
-                              (float)nd[j].m);
-            if (use_quad) {
-              e.b = quad32[2 * (int64_t)v[j]];
-              e.c = quad32[2 * (int64_t)v[j] + 1];
-            } else {
-              e.b = make_float4(0.f, 0.f, 0.f, 0.f);
-              e.c = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            clist[o] = e;
-          }
-        }
-        if (kind == 2) {
-          const float sx = (float)(center[(int64_t)v[j] * DIM] - B.cx);
-          const float sy = (float)(center[(int64_t)v[j] * DIM + 1] - B.cy);
-          const float sz = DIM == 3 ? (float)(center[(int64_t)v[j] * DIM + 2] - B.cz) : 0.f;
-          for (int q = 0; q < nchunk; q++) {
-            const int64_t o = kb + nk + kx + q;
-            if (o < ke) {
-              ChunkEnt e;
-              e.sx = sx;
-              e.sy = sy;
-              e.sz = sz;
-              e.start = nd[j].start + q * ch;
-              e.count = min(ch, nd[j].count - q * ch);
-              e.pad[0] = e.pad[1] = e.pad[2] = 0;
-              klist[o] = e;
-            }
-          }
-        }
-      } else {
-        int pc = kind == 2 ? nd[j].count : 0;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
-        np += pc;
-      }
-      nc += __popcll(bc);
-      nk += ktot;
-      const int push = 2 * __popcll(bo);
-      if (sp + push > stack_cap) {
-        if (lane == 0) atomicOr(err, 1u);
-        return;
-      }
-      sp += push;
     }
+    if (kind == 3) {
+      const int o = sp + 2 * __popcll(bo & lt);
+      if (o + 1 < stack_cap) {
+        stack[o] = nd.fc;
+        stack[o + 1] = nd.fc + 1;
+      }
+    }
+    if (FILL) {
+      if (kind == 1) {
+        const int64_t o = cb + nc + __popcll(bc & lt);
+        if (o < ce) {
+          CellEnt e;
+          e.a = make_float4((float)(nd.cx - B.cx), (float)(nd.cy - B.cy), (float)(nd.cz - B.cz), (float)nd.m);
+          if (use_quad) {
+            e.b = quad32[2 * (int64_t)v];
+            e.c = quad32[2 * (int64_t)v + 1];
+          } else {
+            e.b = make_float4(0.f, 0.f, 0.f, 0.f);
+            e.c = make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+          clist[o] = e;
+        }
+      }
+      if (kind == 2) {
+        const float sx = (float)(nd.cx - B.cx);
+        const float sy = (float)(nd.cy - B.cy);
+        const float sz = DIM == 3 ? (float)(nd.cz - B.cz) : 0.f;
+        for (int q = 0; q < nchunk; q++) {
+          const int64_t o = kb + nk + kx + q;
+          if (o < ke) {
+            ChunkEnt e;
+            e.sx = sx;
+            e.sy = sy;
+            e.sz = sz;
+            e.start = nd.start + q * ch;
+            e.count = min(ch, nd.count - q * ch);
+            e.pad[0] = e.pad[1] = e.pad[2] = 0;
+            klist[o] = e;
+          }
+        }
+      }
+    } else if (kind == 2) {
+      np += nd.count;  // this lane's share; summed over the lanes at the end
+    }
+    nc += __popcll(bc);
+    nk += ktot;
+    const int push = 2 * __popcll(bo);
+    if (sp + push > stack_cap) {
+      if (lane == 0) atomicOr(err, 1u);
+      return;
+    }
+    sp += push;
     wave_sync();
   }
-  if (!FILL && lane == 0) {
-    ccount[g] = nc;
-    kcount[g] = nk;
-    pcount[g] = np;
+  if (!FILL) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) np += __shfl_xor(np, off);
+    if (lane == 0) {
+      ccount[g] = nc;
+      kcount[g] = nk;
+      pcount[g] = np;
+    }
   }
+}
+
+// One launch of the walk for the tree's dimension and the group size (64 or
+// 128 targets: one or two per lane).
+template <bool FILL>
+void launch_walk(bh_ctx* c, const bh_tree* t, dim3 grid, size_t lds, int gs, int use_quad, const GroupBox* boxes,
+                 double theta2, int stack_cap, int ch, int64_t* ccount, int64_t* kcount, int64_t* pcount,
+                 const int64_t* coff, const int64_t* koff, CellEnt* clist, ChunkEnt* klist, uint32_t* err,
+                 const int32_t* groot, int64_t g0, int64_t nlaunch, int wpb, const double4* tpos, int64_t m,
+                 const int32_t* gt0) {
+#define BH_WALK(D, TPW)                                                                                        \
+  hipLaunchKernelGGL((k_group_walk<D, FILL, TPW>), grid, dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,     \
+                     t->quad32, use_quad, boxes, theta2, stack_cap, ch, ccount, kcount, pcount, coff, \
+                     koff, clist, klist, err, groot, g0, nlaunch, wpb, tpos, m, gt0)
+  if (t->dim == 3) {
+    if (gs == 64) BH_WALK(3, 1);
+    else BH_WALK(3, 2);
+  } else {
+    if (gs == 64) BH_WALK(2, 1);
+    else BH_WALK(2, 2);
+  }
+#undef BH_WALK
 }
 
 // ---------------------------------------------------------------- eval
@@ -575,7 +560,7 @@ template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
 __device__ __forceinline__ void eval_item(
     int64_t g, int64_t ta, int64_t tb, int lane, float4 (*tile)[64], int64_t m, int gs,
     const double4* __restrict__ tpos, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
-    const double* __restrict__ center, int dim, const GroupBox* __restrict__ boxes,
+    const double* __restrict__ com, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
     const ChunkEnt* __restrict__ klist, float eps2, const int32_t* __restrict__ gt0, double* __restrict__ part) {
   constexpr int CPT = 64 / CH;
@@ -593,9 +578,9 @@ __device__ __forceinline__ void eval_item(
       // pair has dx == 0 and contributes nothing
       const float4 s = src32[i];
       const int v = pleaf[i];
-      tx[q] = s.x + (float)(center[(int64_t)v * dim] - B.cx);
-      ty[q] = s.y + (float)(center[(int64_t)v * dim + 1] - B.cy);
-      tz[q] = s.z + (dim == 3 ? (float)(center[(int64_t)v * dim + 2] - B.cz) : 0.f);
+      tx[q] = s.x + (float)(com[(int64_t)v * dim] - B.cx);
+      ty[q] = s.y + (float)(com[(int64_t)v * dim + 1] - B.cy);
+      tz[q] = s.z + (dim == 3 ? (float)(com[(int64_t)v * dim + 2] - B.cz) : 0.f);
     } else {
       const double4 p = tpos[i];
       tx[q] = (float)(p.x - B.cx);
@@ -715,7 +700,7 @@ template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
 __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     int64_t g0, int64_t ngroups, const int32_t* __restrict__ ioff, int64_t m, int gs,
     const double4* __restrict__ tpos, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
-    const double* __restrict__ center, int dim, const GroupBox* __restrict__ boxes,
+    const double* __restrict__ com, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
     const ChunkEnt* __restrict__ klist, float eps2, const int32_t* __restrict__ gt0,
     const uint32_t* __restrict__ err, uint32_t* __restrict__ next, double* __restrict__ part) {
@@ -742,7 +727,7 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     const int64_t tiles_g = (coff[g + 1] - coff[g] + 63) / 64 + (koff[g + 1] - koff[g] + CPT - 1) / CPT;
     const int64_t ta = j * ITEM_TILES;
     const int64_t tb = ta + ITEM_TILES < tiles_g ? ta + ITEM_TILES : tiles_g;
-    eval_item<TPL, CH, ZEPS, SELF, KAHAN, QUAD>(g, ta, tb, lane, tiles[w], m, gs, tpos, src32, pleaf, center, dim,
+    eval_item<TPL, CH, ZEPS, SELF, KAHAN, QUAD>(g, ta, tb, lane, tiles[w], m, gs, tpos, src32, pleaf, com, dim,
                                                 boxes, coff, clist, koff, klist, eps2, gt0,
                                                 part + item * (int64_t)(64 * TPL * 3));
   }
@@ -896,7 +881,7 @@ static void launch_eval(bh_ctx* c, int64_t g0, int64_t ng, const int32_t* ioff, 
     const void* kern = reinterpret_cast<const void*>(&k_group_eval<TPL, CH, Z, S, K, Q>);                         \
     hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3(eval_grid(c, kern, items_bound)),                 \
                        dim3(64 * EV_WAVES), 0, c->stream, g0, ng, ioff, m, gs, tpos, t->spart32, t->pleaf,        \
-                       t->center, t->dim, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part);            \
+                       t->com, t->dim, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part);            \
   }
 #define BH_EV_Q(Z, S, K) \
   if (quad) BH_EV(Z, S, K, true) else BH_EV(Z, S, K, false)
@@ -1004,9 +989,6 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_group_box, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, gt0, boxes);
-  SubBox* subs = s.get<SubBox>((size_t)ng * NSUB);
-  if (!subs) return BH_ENOMEM;
-  hipLaunchKernelGGL(k_group_sub, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, gt0, subs);
   const double theta2 = theta * theta;
   // wavefronts per workgroup: WALK_WAVES unless their stacks exceed the
   // workgroup's LDS limit
@@ -1020,18 +1002,13 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   // the count pass; an LDS stack overflow on the host-scanned path reruns it
   // with twice the stack (the lists do not depend on the stack size)
   auto count_pass = [&]() -> int {
-    const int64_t per_wave = (int64_t)stack_cap * 4 + (int64_t)(NSUB + gs / 2) * (int64_t)sizeof(SubBox);
+    const int64_t per_wave = (int64_t)stack_cap * 4;
     wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / per_wave));
     lds = (size_t)wpb * per_wave;
     ProfScope p(c, "k_group_walk_count");
-    if (t->dim == 3)
-      hipLaunchKernelGGL((k_group_walk<3, false, WALK_W>), dim3((unsigned)cdiv(ng, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes, t->center,
-                         t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb, tpos, m, gs, gt0, subs);
-    else
-      hipLaunchKernelGGL((k_group_walk<2, false, WALK_W>), dim3((unsigned)cdiv(ng, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes, t->center,
-                         t->quad32, (int)quad, boxes, theta2, stack_cap, ch, cnt, cnt + ng, cnt + 2 * ng, nullptr,
-                         nullptr, nullptr, nullptr, err, groot, (int64_t)0, ng, wpb, tpos, m, gs, gt0, subs);
+    launch_walk<false>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, (int)quad, boxes, theta2, stack_cap, ch, cnt,
+                       cnt + ng, cnt + 2 * ng, nullptr, nullptr, nullptr, nullptr, err, groot, 0, ng, wpb, tpos, m,
+                       gt0);
     BH_LAUNCH_CHECK();
     return BH_OK;
   };
@@ -1150,14 +1127,9 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     }
     {
       ProfScope p(c, "k_group_walk_fill");
-      if (t->dim == 3)
-        hipLaunchKernelGGL((k_group_walk<3, true, WALK_W>), dim3((unsigned)cdiv(nb, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,
-                           t->center, t->quad32, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
-                           offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos, m, gs, gt0, subs);
-      else
-        hipLaunchKernelGGL((k_group_walk<2, true, WALK_W>), dim3((unsigned)cdiv(nb, wpb)), dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,
-                           t->center, t->quad32, (int)quad, boxes, theta2, stack_cap, ch, nullptr, nullptr, nullptr,
-                           offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos, m, gs, gt0, subs);
+      launch_walk<true>(c, t, dim3((unsigned)cdiv(nb, wpb)), lds, gs, (int)quad, boxes, theta2, stack_cap, ch,
+                        nullptr, nullptr, nullptr, offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos,
+                        m, gt0);
       BH_LAUNCH_CHECK();
     }
     {
