@@ -247,24 +247,20 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int MAX_STACK = 8192;  // LDS stack entries per wavefront, at most
 constexpr int WALK_WAVES = 4;    // groups (wavefronts) per workgroup, each with its own LDS stack
-
-// The double held by lane j of the wavefront (j uniform).
-__device__ __forceinline__ double readlane_d(double v, int j) {
-  const int64_t b = __double_as_longlong(v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)b, j);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(b >> 32), j);
-  return __longlong_as_double((int64_t)(((uint64_t)hi << 32) | lo));
-}
+constexpr int CBUF = 16;         // candidate nodes per round of the exact test (LDS, per wavefront)
 
 // Each step pops up to 64 stack entries, one per lane, and tests those nodes
 // at once; the output order (step, lane) is the same in the count and fill
 // passes.  TPW = group size / 64 targets per lane: the exact test runs
 // across the lanes, every lane measuring its own targets against one
-// candidate node at a time.
+// candidate node at a time, the candidates read from LDS as broadcasts.
+// Opening radius: a node is accepted for a point at squared distance d2 from
+// its centre of mass iff size^2 / theta^2 < d2 (itheta2 = 1 / theta^2; at
+// theta = 0 it is +inf and every node opens).
 template <int DIM, bool FILL, int TPW>
 __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __restrict__ nodes,
                                                    const float4* __restrict__ quad32, int use_quad,
-                                                   const GroupBox* __restrict__ boxes, double theta2,
+                                                   const GroupBox* __restrict__ boxes, double itheta2,
                                                    int stack_cap, int ch, int64_t* __restrict__ ccount,
                                                    int64_t* __restrict__ kcount, int64_t* __restrict__ pcount,
                                                    const int64_t* __restrict__ coff,
@@ -283,6 +279,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
   const int64_t g = g0 + gl;
   const int lane = threadIdx.x % 64;
   int32_t* stack = lds_stacks + wv * stack_cap;
+  double4* cbuf = reinterpret_cast<double4*>(lds_stacks + wpb * stack_cap) + wv * CBUF;
   int64_t t0, t1;
   group_range(g, m, 64 * TPW, gt0, t0, t1);
   // this lane's targets; lanes past the group's end repeat its first target,
@@ -297,6 +294,9 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
   const GroupBox B = boxes[g];
   if (lane == 0) stack[0] = groot ? groot[g] : 0;  // the group's own system
   wave_sync();
+  // the targets have arrived before the loop, so no wait inside it (where
+  // the list stores of earlier steps are outstanding) is for them
+  __builtin_amdgcn_s_waitcnt(0);
   int sp = 1;
   int64_t nc = 0, nk = 0, np = 0;
   const int64_t cb = FILL ? coff[g] : 0, ce = FILL ? coff[g + 1] : 0;
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
     if (v >= 0) nd = nodes[v];
     int kind = 0;  // 1 cell, 2 leaf, 3 open
     int nchunk = 0;
-    const double s2 = nd.size * nd.size;
+    const double r2 = nd.size * nd.size * itheta2;  // opening radius^2
     if (v >= 0 && nd.m != 0.0) {
       if (nd.fc < 0) {
         kind = 2;
@@ -323,8 +323,8 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
         ex = ex > 0.0 ? ex : 0.0;
         ey = ey > 0.0 ? ey : 0.0;
         ez = ez > 0.0 ? ez : 0.0;
-        const double d2 = ex * ex + ey * ey + ez * ez;
-        const bool open = !(d2 > 0.0) || !(s2 < theta2 * d2);
+        const double d2 = fma(ex, ex, fma(ey, ey, ez * ez));
+        const bool open = !(d2 > 0.0) || !(r2 < d2);
         kind = open ? 3 : 1;
         if (open) {
           // the box test opens; the exact test -- every target of the
@@ -332,28 +332,34 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
           // still accept when the box is loose around its targets.  Its
           // necessary condition uses the first target.
           const double ux = T0.x - nd.cx, uy = T0.y - nd.cy, uz = DIM == 3 ? T0.z - nd.cz : 0.0;
-          if (s2 < theta2 * (ux * ux + uy * uy + uz * uz)) kind = 4;
+          if (r2 < fma(ux, ux, fma(uy, uy, uz * uz))) kind = 4;
         }
       }
     }
     // exact test, one candidate node at a time across the lanes: accept iff
-    // no target lies within size/theta of the centre of mass
+    // no target lies within the opening radius of the centre of mass.  The
+    // candidates go through LDS in rounds of CBUF, in lane order.
     uint64_t cand = __ballot(kind == 4);
     if (cand) {
       uint64_t nearm = 0;
       do {
-        const int j = __builtin_ctzll(cand);
-        cand &= cand - 1;
-        const double cx = readlane_d(nd.cx, j), cy = readlane_d(nd.cy, j);
-        const double cz = DIM == 3 ? readlane_d(nd.cz, j) : 0.0;
-        const double sj = readlane_d(s2, j);
-        bool near = false;
+        const int r = __popcll(cand & lt);
+        if (((cand >> lane) & 1) && r < CBUF) cbuf[r] = make_double4(nd.cx, nd.cy, DIM == 3 ? nd.cz : 0.0, r2);
+        wave_sync();
+        const int nb = __popcll(cand) < CBUF ? __popcll(cand) : CBUF;
+        for (int i = 0; i < nb; i++) {
+          const int j = __builtin_ctzll(cand);
+          cand &= cand - 1;
+          const double4 cn = cbuf[i];
+          bool near = false;
 #pragma unroll
-        for (int q = 0; q < TPW; q++) {
-          const double ux = tq[q].x - cx, uy = tq[q].y - cy, uz = DIM == 3 ? tq[q].z - cz : 0.0;
-          near = near || !(sj < theta2 * (ux * ux + uy * uy + uz * uz));
+          for (int q = 0; q < TPW; q++) {
+            const double ux = tq[q].x - cn.x, uy = tq[q].y - cn.y, uz = DIM == 3 ? tq[q].z - cn.z : 0.0;
+            near = near || !(cn.w < fma(ux, ux, fma(uy, uy, uz * uz)));
+          }
+          if (__ballot(near)) nearm |= 1ull << j;
         }
-        if (__ballot(near)) nearm |= 1ull << j;
+        wave_sync();  // every read of this round precedes the next round's writes
       } while (cand);
       if (kind == 4) kind = (nearm >> lane) & 1 ? 3 : 1;
     }
@@ -444,13 +450,13 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
 // 128 targets: one or two per lane).
 template <bool FILL>
 void launch_walk(bh_ctx* c, const bh_tree* t, dim3 grid, size_t lds, int gs, int use_quad, const GroupBox* boxes,
-                 double theta2, int stack_cap, int ch, int64_t* ccount, int64_t* kcount, int64_t* pcount,
+                 double itheta2, int stack_cap, int ch, int64_t* ccount, int64_t* kcount, int64_t* pcount,
                  const int64_t* coff, const int64_t* koff, CellEnt* clist, ChunkEnt* klist, uint32_t* err,
                  const int32_t* groot, int64_t g0, int64_t nlaunch, int wpb, const double4* tpos, int64_t m,
                  const int32_t* gt0) {
 #define BH_WALK(D, TPW)                                                                                        \
   hipLaunchKernelGGL((k_group_walk<D, FILL, TPW>), grid, dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,     \
-                     t->quad32, use_quad, boxes, theta2, stack_cap, ch, ccount, kcount, pcount, coff, \
+                     t->quad32, use_quad, boxes, itheta2, stack_cap, ch, ccount, kcount, pcount, coff, \
                      koff, clist, klist, err, groot, g0, nlaunch, wpb, tpos, m, gt0)
   if (t->dim == 3) {
     if (gs == 64) BH_WALK(3, 1);
@@ -989,7 +995,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_group_box, dim3((unsigned)ng), dim3(64), 0, c->stream, tpos, m, gs, gt0, boxes);
-  const double theta2 = theta * theta;
+  const double itheta2 = theta > 0.0 ? 1.0 / (theta * theta) : __builtin_inf();
   // wavefronts per workgroup: WALK_WAVES unless their stacks exceed the
   // workgroup's LDS limit
   int max_lds = 65536;
@@ -1002,11 +1008,11 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   // the count pass; an LDS stack overflow on the host-scanned path reruns it
   // with twice the stack (the lists do not depend on the stack size)
   auto count_pass = [&]() -> int {
-    const int64_t per_wave = (int64_t)stack_cap * 4;
+    const int64_t per_wave = (int64_t)stack_cap * 4 + CBUF * (int64_t)sizeof(double4);
     wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / per_wave));
     lds = (size_t)wpb * per_wave;
     ProfScope p(c, "k_group_walk_count");
-    launch_walk<false>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, (int)quad, boxes, theta2, stack_cap, ch, cnt,
+    launch_walk<false>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, (int)quad, boxes, itheta2, stack_cap, ch, cnt,
                        cnt + ng, cnt + 2 * ng, nullptr, nullptr, nullptr, nullptr, err, groot, 0, ng, wpb, tpos, m,
                        gt0);
     BH_LAUNCH_CHECK();
@@ -1127,7 +1133,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     }
     {
       ProfScope p(c, "k_group_walk_fill");
-      launch_walk<true>(c, t, dim3((unsigned)cdiv(nb, wpb)), lds, gs, (int)quad, boxes, theta2, stack_cap, ch,
+      launch_walk<true>(c, t, dim3((unsigned)cdiv(nb, wpb)), lds, gs, (int)quad, boxes, itheta2, stack_cap, ch,
                         nullptr, nullptr, nullptr, offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos,
                         m, gt0);
       BH_LAUNCH_CHECK();
