@@ -269,14 +269,16 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
                                                    uint32_t* __restrict__ err, const int32_t* __restrict__ groot,
                                                    int64_t g0, int64_t nlaunch, int wpb,
                                                    const double4* __restrict__ tpos, int64_t m,
-                                                   const int32_t* __restrict__ gt0) {
+                                                   const int32_t* __restrict__ gt0,
+                                                   const int32_t* __restrict__ order) {
   extern __shared__ int32_t lds_stacks[];
   // lists of fixed capacity that would overflow are not written at all
   if (FILL && (__builtin_amdgcn_readfirstlane(*err) & 2u)) return;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const int64_t gl = (int64_t)blockIdx.x * wpb + wv;
   if (wv >= wpb || gl >= nlaunch) return;  // whole wavefront
-  const int64_t g = g0 + gl;
+  // order: launch slot -> group (relative to g0), longest lists first
+  const int64_t g = g0 + (order ? (int64_t)order[gl] : gl);
   const int lane = threadIdx.x % 64;
   int32_t* stack = lds_stacks + wv * stack_cap;
   double4* cbuf = reinterpret_cast<double4*>(lds_stacks + wpb * stack_cap) + wv * CBUF;
@@ -453,11 +455,11 @@ void launch_walk(bh_ctx* c, const bh_tree* t, dim3 grid, size_t lds, int gs, int
                  double itheta2, int stack_cap, int ch, int64_t* ccount, int64_t* kcount, int64_t* pcount,
                  const int64_t* coff, const int64_t* koff, CellEnt* clist, ChunkEnt* klist, uint32_t* err,
                  const int32_t* groot, int64_t g0, int64_t nlaunch, int wpb, const double4* tpos, int64_t m,
-                 const int32_t* gt0) {
+                 const int32_t* gt0, const int32_t* order) {
 #define BH_WALK(D, TPW)                                                                                        \
   hipLaunchKernelGGL((k_group_walk<D, FILL, TPW>), grid, dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,     \
                      t->quad32, use_quad, boxes, itheta2, stack_cap, ch, ccount, kcount, pcount, coff, \
-                     koff, clist, klist, err, groot, g0, nlaunch, wpb, tpos, m, gt0)
+                     koff, clist, klist, err, groot, g0, nlaunch, wpb, tpos, m, gt0, order)
   if (t->dim == 3) {
     if (gs == 64) BH_WALK(3, 1);
     else BH_WALK(3, 2);
@@ -1014,7 +1016,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     ProfScope p(c, "k_group_walk_count");
     launch_walk<false>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, (int)quad, boxes, itheta2, stack_cap, ch, cnt,
                        cnt + ng, cnt + 2 * ng, nullptr, nullptr, nullptr, nullptr, err, groot, 0, ng, wpb, tpos, m,
-                       gt0);
+                       gt0, nullptr);
     BH_LAUNCH_CHECK();
     return BH_OK;
   };
@@ -1110,10 +1112,34 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const bool zeps = !(eps2 > 0.f);
   const bool self = tgt == nullptr;
   std::vector<int64_t> rel;
-  std::vector<int32_t> reli;
+  std::vector<int32_t> reli, hord, bucket;
+  int32_t* ord = nullptr;
+  if (!plan) {
+    int64_t maxnb = 0;
+    for (const Batch& b : batches) maxnb = std::max(maxnb, b.g1 - b.g0);
+    ord = s.get<int32_t>((size_t)std::max<int64_t>(maxnb, 1));
+    if (!ord) return BH_ENOMEM;
+  }
   for (const Batch& b : batches) {
     const int64_t nb = b.g1 - b.g0;
     if (!plan) {
+      // fill order: groups with the longest lists (the longest walks) first,
+      // so the pass does not end on a few long walks started late; a
+      // counting sort on the list length, in 4096 buckets
+      int64_t kmax = 1;
+      for (int64_t g = b.g0; g < b.g1; g++) kmax = std::max(kmax, hc[g] + hc[ng + g]);
+      int sh = 0;
+      while ((kmax >> sh) >= 4096) sh++;
+      bucket.assign(4097, 0);
+      for (int64_t g = b.g0; g < b.g1; g++) bucket[4095 - ((hc[g] + hc[ng + g]) >> sh)]++;
+      for (int k = 0, acc = 0; k <= 4096; k++) {
+        const int cnt_k = bucket[k];
+        bucket[k] = acc;
+        acc += cnt_k;
+      }
+      hord.resize(nb);
+      for (int64_t g = b.g0; g < b.g1; g++) hord[bucket[4095 - ((hc[g] + hc[ng + g]) >> sh)]++] = (int32_t)(g - b.g0);
+      BH_HIP(hipMemcpyAsync(ord, hord.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
       // offsets of this batch's groups and items, relative to its first
       // list entries and first item
       rel.resize(2 * (nb + 1));
@@ -1129,13 +1155,13 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       BH_HIP(hipMemcpyAsync(offs + ng + 1 + b.g0, rel.data() + nb + 1, (nb + 1) * 8, hipMemcpyHostToDevice,
                             c->stream));
       BH_HIP(hipMemcpyAsync(ioff, reli.data(), (nb + 1) * 4, hipMemcpyHostToDevice, c->stream));
-      BH_HIP(hipStreamSynchronize(c->stream));  // rel, reli are reused by the next batch
+      BH_HIP(hipStreamSynchronize(c->stream));  // rel, reli, hord are reused by the next batch
     }
     {
       ProfScope p(c, "k_group_walk_fill");
       launch_walk<true>(c, t, dim3((unsigned)cdiv(nb, wpb)), lds, gs, (int)quad, boxes, itheta2, stack_cap, ch,
                         nullptr, nullptr, nullptr, offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos,
-                        m, gt0);
+                        m, gt0, ord);
       BH_LAUNCH_CHECK();
     }
     {
