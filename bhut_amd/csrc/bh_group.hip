@@ -632,56 +632,110 @@ __device__ __forceinline__ void eval_item(
     }
   }
 
-  // ---- leaf chunks of tiles [max(ta, ctiles), tb) - ctiles: lane l holds
-  // particle l%CH of chunk l/CH (padding slots carry zero mass far away)
+  // ---- leaf chunks of tiles [max(ta, ctiles), tb) - ctiles, compacted: the
+  // item's chunks form one stream of sources cut into tiles of 64, so a
+  // chunk of fewer than CH particles leaves no idle slots (only the item's
+  // last tile is partly filled).  Chunks are staged in windows of 64 in the
+  // LDS tiles the cell part used: entries (shift xyz, first source) in
+  // tile[1], the exclusive prefix of their counts in tile[2]; each lane finds
+  // the chunk of its stream position by a binary search of that prefix.
   const int64_t k0 = koff[g], k1 = koff[g + 1];
   const int64_t ka0 = ta > ctiles ? ta - ctiles : 0;
   const int64_t kb0 = tb - ctiles;
   const int64_t ks = k0 + CPT * ka0;
   const int64_t ke = k0 + CPT * kb0 < k1 ? k0 + CPT * kb0 : k1;
   if (kb0 > 0 && ks < ke) {
-    // Two-stage pipeline over tiles: the chunk entries of tile i+2 and the
-    // sources of tile i+1 are in flight while tile i is consumed; the shift
-    // is applied only when a tile is written to LDS, so no load result is
-    // needed before its tile comes up.
-    const int sub = lane / CH, off = lane % CH;
     const float4* kl4 = reinterpret_cast<const float4*>(klist);  // ChunkEnt = 2 x 16 B
-    // raw chunk fields: shift xyz + start bits in .w, count bits in .x
-    auto load_chunk = [&](int64_t e, float4& kx, float4& ky) {
-      const int64_t ce2 = e + sub;
-      if (ce2 < ke) {
-        kx = kl4[2 * ce2];
-        ky = kl4[2 * ce2 + 1];
-      } else {
-        kx = make_float4(0.f, 0.f, 0.f, 0.f);
-        ky = make_float4(0.f, 0.f, 0.f, 0.f);  // count 0: every slot is padding
+    float4* went = tile[1];                                    // (shift x, y, z, first source bits)
+    int* wpre = reinterpret_cast<int*>(tile[2]);                // [0..64]
+    int64_t w = ks;  // first chunk of the window
+    int nw = 0;      // chunks in the window
+    int wtot = 0;    // sources in the window
+    int tp = 0;      // stream position of the next tile, relative to the window
+    auto load_window = [&]() {
+      const int64_t e = w + lane;
+      float4 ex = make_float4(0.f, 0.f, 0.f, 0.f);
+      int cnt = 0;
+      if (e < ke) {
+        ex = kl4[2 * e];
+        cnt = __float_as_int(kl4[2 * e + 1].x);
       }
-    };
-    auto src_index = [&](const float4& kx, const float4& ky) -> int {
-      return off < __float_as_int(ky.x) ? __float_as_int(kx.w) + off : -1;
-    };
-    auto load_src = [&](int kidx) -> float4 {
-      float4 v = make_float4(1e15f, 1e15f, 1e15f, 0.f);
-      if (kidx >= 0) v = src32[kidx];
-      return v;
-    };
-    float4 ka_cur, kb_cur, ka_nxt, kb_nxt;
-    load_chunk(ks, ka_cur, kb_cur);
-    int idx_cur = src_index(ka_cur, kb_cur);
-    float4 src_cur = load_src(idx_cur);
-    load_chunk(ks + CPT, ka_nxt, kb_nxt);
-    for (int64_t e = ks; e < ke; e += CPT) {
+      int inc = cnt;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+      }
+      wave_sync();  // every read of the previous window precedes these writes
+      went[lane] = ex;
+      wpre[lane] = inc - cnt;
       wave_sync();
-      tile[0][lane] = idx_cur >= 0 ? make_float4(src_cur.x + ka_cur.x, src_cur.y + ka_cur.y,
-                                                 src_cur.z + ka_cur.z, src_cur.w)
-                                   : make_float4(1e15f, 1e15f, 1e15f, 0.f);
+      nw = ke - w < 64 ? (int)(ke - w) : 64;
+      wtot = __builtin_amdgcn_readfirstlane(__shfl(inc, 63));
+    };
+    // the source at stream position P of the window (0 <= P < wtot): the
+    // last chunk i < nw whose prefix is <= P
+    auto locate = [&](int P, float4& sh, int& idx) {
+      int lo = 0, plo = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {
+        if (lo + step < nw) {
+          const int q = wpre[lo + step];
+          if (q <= P) {
+            lo += step;
+            plo = q;
+          }
+        }
+      }
+      sh = went[lo];
+      idx = __float_as_int(sh.w) + (P - plo);
+    };
+    // the next tile: each lane's source (load in flight) and shift; returns
+    // the number of filled slots (lanes 0 .. n-1)
+    auto map_tile = [&](float4& src, float4& sh, bool& valid) -> int {
+      if (tp >= wtot && w + nw < ke) {
+        tp -= wtot;
+        w += nw;
+        load_window();
+      }
+      int P = tp + lane;
+      int idx = 0;
+      valid = false;
+      if (P < wtot) {
+        locate(P, sh, idx);
+        valid = true;
+      }
+      if (tp + 64 > wtot && w + nw < ke) {
+        // the tile continues in the next window (a window of 64 chunks holds
+        // at least 64 sources, so no tile spans three)
+        const int old = wtot;
+        w += nw;
+        load_window();
+        tp -= old;
+        P -= old;
+        if (!valid && P >= 0 && P < wtot) {
+          locate(P, sh, idx);
+          valid = true;
+        }
+      }
+      tp += 64;
+      src = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (valid) src = src32[idx];
+      return __popcll(__ballot(valid));
+    };
+    load_window();
+    float4 src_cur, sh_cur;
+    bool v_cur;
+    int n_cur = map_tile(src_cur, sh_cur, v_cur);
+    for (;;) {
       wave_sync();
-      ka_cur = ka_nxt;
-      kb_cur = kb_nxt;
-      idx_cur = src_index(ka_cur, kb_cur);
-      src_cur = load_src(idx_cur);
-      load_chunk(e + 2 * CPT, ka_nxt, kb_nxt);
-      const int nslots = (int)((ke - e < CPT ? ke - e : CPT) * CH);
+      tile[0][lane] = v_cur ? make_float4(src_cur.x + sh_cur.x, src_cur.y + sh_cur.y, src_cur.z + sh_cur.z,
+                                          src_cur.w)
+                            : make_float4(1e15f, 1e15f, 1e15f, 0.f);
+      wave_sync();
+      const int nslots = n_cur;
+      const bool more = !(tp >= wtot && w + nw >= ke);
+      if (more) n_cur = map_tile(src_cur, sh_cur, v_cur);  // the next tile's loads are in flight
 #pragma unroll 8
       for (int k = 0; k < nslots; k++) {
         const float4 v = tile[0][k];
@@ -689,6 +743,7 @@ __device__ __forceinline__ void eval_item(
         for (int q = 0; q < TPL; q++) p2p<ZEPS>(v, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
       }
       flush<TPL, KAHAN>(px, py, pz, sx, sy, sz, cx, cy, cz);
+      if (!more) break;
     }
   }
 #pragma unroll
