@@ -17,7 +17,8 @@
 //     Run twice: count, then fill.
 //  3. k_group_eval: persistent wavefronts over work items (a group's list
 //     cut into pieces of at most 64 tiles), one (or two) targets per lane.
-//     Tiles of 64 list entries are loaded cooperatively (coalesced) into one
+//     Tiles of 64 cells, or of 64 sources of the item's leaf chunks taken as
+//     one compacted stream, are loaded cooperatively (coalesced) into one
 //     LDS tile per wavefront and read back as wave-wide broadcasts; the next
 //     tile's HBM reads are in flight in registers while the current tile is
 //     consumed.  k_item_reduce adds each target's item partials in item
