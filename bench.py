@@ -70,6 +70,8 @@ def parse():
     p.add_argument("--systems", type=int, default=512, help="ensemble: number of systems")
     p.add_argument("--lf-steps", type=int, default=100, help="leapfrog: steps")
     p.add_argument("--no-graph", action="store_true", help="leapfrog: eager steps instead of graphs")
+    p.add_argument("--list-budget-gib", type=float, default=0.0,
+                   help="FP32 form: bytes of interaction lists held at once, GiB (0: the library default)")
     return p.parse_args()
 
 
@@ -545,6 +547,8 @@ def main():
               "described in DESIGN.md and not measured here", file=sys.stderr)
         sys.exit(2)
     env = Env()
+    if args.list_budget_gib > 0:
+        env.ctx.set_list_budget(int(args.list_budget_gib * 2 ** 30))
     fn = {"plummer": workload_plummer, "ensemble": workload_ensemble, "diskhalo": workload_diskhalo,
           "leapfrog": workload_leapfrog, "copy": workload_copy}[args.workload]
     line = fn(args, env)

@@ -289,7 +289,10 @@ int bh_ctx_create(int device, bh_ctx** out) {
   }
   c->stream = c->own_stream;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    c->num_cus = prop.multiProcessorCount;
+    c->mem_total = prop.totalGlobalMem;
+  }
   *out = c;
   return BH_OK;
 }

@@ -80,6 +80,7 @@ struct bh_ctx {
   hipStream_t stream = nullptr;      // stream all work is enqueued on
   hipStream_t own_stream = nullptr;  // the context's own (default-priority) stream
   int num_cus = 0;
+  size_t mem_total = 0;  // device memory, bytes
   // caching device allocator: exact-size free lists
   std::mutex mu;
   std::multimap<size_t, void*> free_blocks;

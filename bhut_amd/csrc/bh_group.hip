@@ -908,8 +908,12 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
 
 // ---------------------------------------------------------------- host
 // Bytes of interaction lists materialised at once by the host-scanned path
-// unless the context sets another budget.
-constexpr size_t LIST_BUDGET = size_t(12) << 30;
+// unless the context sets another budget: a quarter of the device memory, at
+// most 64 GiB (12 GiB when the device memory is unknown).
+static size_t default_list_budget(const bh_ctx* c) {
+  if (c->mem_total == 0) return size_t(12) << 30;
+  return std::min<size_t>(size_t(64) << 30, c->mem_total / 4);
+}
 
 // Workgroups of the persistent evaluation: all that fit at once, but no
 // more than the groups need.
@@ -1122,7 +1126,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       if (!rc) rc = count_pass();
       if (rc) return rc;
     }
-    const size_t budget = c->list_budget ? c->list_budget : LIST_BUDGET;
+    const size_t budget = c->list_budget ? c->list_budget : default_list_budget(c);
     ho.assign(2 * (ng + 1), 0);
     int64_t maxc = 0, maxk = 0, bc = 0, bk = 0, bi = 0;
     size_t bytes = 0;

@@ -58,7 +58,8 @@ int bh_ctx_set_stream(bh_ctx *ctx, void *stream);
 int bh_ctx_trim(bh_ctx *ctx);                 /* release cached allocations  */
 /* Upper bound, in bytes, on the FP32 form's interaction lists held at once;
  * larger problems are walked and evaluated in batches of groups.  0 selects
- * the default (12 GiB).  Results do not depend on it. */
+ * the default (a quarter of the device memory, at most 64 GiB).  Results do
+ * not depend on it. */
 int bh_ctx_set_list_budget(bh_ctx *ctx, int64_t bytes);
 /* Per-kernel timing with HIP events recorded on the context stream around
  * the hot kernels (walks, list builds, evaluations).  enable=1 clears the
