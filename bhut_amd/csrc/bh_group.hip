@@ -41,21 +41,13 @@ struct GroupBox {
   double cx, cy, cz, hx, hy, hz;
 };
 
-// Accepted cell: com - group centre, mass; second moments about the com.
-struct __attribute__((aligned(16))) CellEnt {
-  float4 a;  // x, y, z, m
-  float4 b;  // qxx, qyy, qzz, trace
-  float4 c;  // qxy, qxz, qyz, 0
-};
-
-// A run of at most CH particles of one leaf.
-struct __attribute__((aligned(16))) ChunkEnt {
-  float sx, sy, sz;  // leaf centre of mass - group centre
-  int32_t start;     // first particle (sorted order)
-  int32_t count;     // particles in this chunk
-  int32_t pad[3];
-};
-static_assert(sizeof(ChunkEnt) == 32, "ChunkEnt layout");
+// List entries are node indices: an accepted cell (its centre of mass, mass
+// and second moments are read from the node record and the FP32 moment
+// payload when the cell is evaluated), or a leaf (its particle range and
+// centre of mass likewise).  4 B per entry keeps the lists small next to the
+// tree, so the walk writes and the evaluation reads little HBM.
+using CellEnt = int32_t;
+using ChunkEnt = int32_t;
 
 // ---------------------------------------------------------------- prepare
 // Sources relative to their leaf's centre of mass (FP32) and each
@@ -260,9 +252,8 @@ constexpr int CBUF = 16;         // candidate nodes per round of the exact test 
 // theta = 0 it is +inf and every node opens).
 template <int DIM, bool FILL, int TPW>
 __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __restrict__ nodes,
-                                                   const float4* __restrict__ quad32, int use_quad,
                                                    const GroupBox* __restrict__ boxes, double itheta2,
-                                                   int stack_cap, int ch, int64_t* __restrict__ ccount,
+                                                   int stack_cap, int64_t* __restrict__ ccount,
                                                    int64_t* __restrict__ kcount, int64_t* __restrict__ pcount,
                                                    const int64_t* __restrict__ coff,
                                                    const int64_t* __restrict__ koff,
@@ -313,12 +304,10 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
     NodeD nd;
     if (v >= 0) nd = nodes[v];
     int kind = 0;  // 1 cell, 2 leaf, 3 open
-    int nchunk = 0;
     const double r2 = nd.size * nd.size * itheta2;  // opening radius^2
     if (v >= 0 && nd.m != 0.0) {
       if (nd.fc < 0) {
         kind = 2;
-        nchunk = (nd.count + ch - 1) / ch;
       } else {
         double ex = fabs(nd.cx - B.cx) - B.hx;
         double ey = fabs(nd.cy - B.cy) - B.hy;
@@ -367,23 +356,8 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
       if (kind == 4) kind = (nearm >> lane) & 1 ? 3 : 1;
     }
     const uint64_t bc = __ballot(kind == 1), bo = __ballot(kind == 3);
-    // exclusive prefix of chunk counts over the lanes: ballot counts when no
-    // leaf needs more than one chunk (leaf_size <= CH), else a wave scan
-    int kx, ktot;
-    if (__ballot(nchunk > 1) == 0) {
-      const uint64_t bk = __ballot(nchunk == 1);
-      kx = __popcll(bk & lt);
-      ktot = __popcll(bk);
-    } else {
-      kx = nchunk;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(kx, off);
-        if (lane >= off) kx += y;
-      }
-      ktot = __shfl(kx, 63);
-      kx -= nchunk;
-    }
+    const uint64_t bk = __ballot(kind == 2);
+    const int kx = __popcll(bk & lt), ktot = __popcll(bk);
     if (kind == 3) {
       const int o = sp + 2 * __popcll(bo & lt);
       if (o + 1 < stack_cap) {
@@ -394,36 +368,11 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
     if (FILL) {
       if (kind == 1) {
         const int64_t o = cb + nc + __popcll(bc & lt);
-        if (o < ce) {
-          CellEnt e;
-          e.a = make_float4((float)(nd.cx - B.cx), (float)(nd.cy - B.cy), (float)(nd.cz - B.cz), (float)nd.m);
-          if (use_quad) {
-            e.b = quad32[2 * (int64_t)v];
-            e.c = quad32[2 * (int64_t)v + 1];
-          } else {
-            e.b = make_float4(0.f, 0.f, 0.f, 0.f);
-            e.c = make_float4(0.f, 0.f, 0.f, 0.f);
-          }
-          clist[o] = e;
-        }
+        if (o < ce) clist[o] = v;
       }
       if (kind == 2) {
-        const float sx = (float)(nd.cx - B.cx);
-        const float sy = (float)(nd.cy - B.cy);
-        const float sz = DIM == 3 ? (float)(nd.cz - B.cz) : 0.f;
-        for (int q = 0; q < nchunk; q++) {
-          const int64_t o = kb + nk + kx + q;
-          if (o < ke) {
-            ChunkEnt e;
-            e.sx = sx;
-            e.sy = sy;
-            e.sz = sz;
-            e.start = nd.start + q * ch;
-            e.count = min(ch, nd.count - q * ch);
-            e.pad[0] = e.pad[1] = e.pad[2] = 0;
-            klist[o] = e;
-          }
-        }
+        const int64_t o = kb + nk + kx;
+        if (o < ke) klist[o] = v;
       }
     } else if (kind == 2) {
       np += nd.count;  // this lane's share; summed over the lanes at the end
@@ -452,14 +401,14 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
 // One launch of the walk for the tree's dimension and the group size (64 or
 // 128 targets: one or two per lane).
 template <bool FILL>
-void launch_walk(bh_ctx* c, const bh_tree* t, dim3 grid, size_t lds, int gs, int use_quad, const GroupBox* boxes,
-                 double itheta2, int stack_cap, int ch, int64_t* ccount, int64_t* kcount, int64_t* pcount,
+void launch_walk(bh_ctx* c, const bh_tree* t, dim3 grid, size_t lds, int gs, const GroupBox* boxes,
+                 double itheta2, int stack_cap, int64_t* ccount, int64_t* kcount, int64_t* pcount,
                  const int64_t* coff, const int64_t* koff, CellEnt* clist, ChunkEnt* klist, uint32_t* err,
                  const int32_t* groot, int64_t g0, int64_t nlaunch, int wpb, const double4* tpos, int64_t m,
                  const int32_t* gt0, const int32_t* order) {
 #define BH_WALK(D, TPW)                                                                                        \
   hipLaunchKernelGGL((k_group_walk<D, FILL, TPW>), grid, dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,     \
-                     t->quad32, use_quad, boxes, itheta2, stack_cap, ch, ccount, kcount, pcount, coff, \
+                     boxes, itheta2, stack_cap, ccount, kcount, pcount, coff,                       \
                      koff, clist, klist, err, groot, g0, nlaunch, wpb, tpos, m, gt0, order)
   if (t->dim == 3) {
     if (gs == 64) BH_WALK(3, 1);
@@ -569,6 +518,7 @@ template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
 __device__ __forceinline__ void eval_item(
     int64_t g, int64_t ta, int64_t tb, int lane, float4 (*tile)[64], int64_t m, int gs,
     const double4* __restrict__ tpos, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
+    const NodeD* __restrict__ nodes, const float4* __restrict__ quad32,
     const double* __restrict__ com, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
     const ChunkEnt* __restrict__ klist, float eps2, const int32_t* __restrict__ gt0, double* __restrict__ part) {
@@ -601,28 +551,35 @@ __device__ __forceinline__ void eval_item(
   const int64_t c0 = coff[g], c1 = coff[g + 1];
   const int64_t ctiles = (c1 - c0 + 63) / 64;
 
-  // ---- accepted cells of tiles [ta, min(tb, ctiles))
+  // ---- accepted cells of tiles [ta, min(tb, ctiles)): each lane gathers
+  // one cell's node record and moment payload; the indices are loaded two
+  // tiles ahead and the payloads one tile ahead of the tile being consumed
   if (ta < ctiles) {
     const int64_t cs = c0 + 64 * ta;
     const int64_t ce = tb < ctiles ? c0 + 64 * tb : c1;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 na = z4, nb = z4, nc = z4;
-    if (cs + lane < ce) {
-      na = clist[cs + lane].a;
-      nb = clist[cs + lane].b;
-      nc = clist[cs + lane].c;
-    }
+    double4 na = make_double4(0.0, 0.0, 0.0, 0.0);  // centre of mass, mass
+    float4 nb = z4, nc = z4;
+    auto load_cell = [&](int v) {
+      if (v >= 0) {
+        const double4* r = reinterpret_cast<const double4*>(nodes + v);
+        na = r[0];
+        if (QUAD) {
+          nb = quad32[2 * (int64_t)v];
+          nc = quad32[2 * (int64_t)v + 1];
+        }
+      }
+    };
+    int v_nxt = cs + 64 + lane < ce ? clist[cs + 64 + lane] : -1;
+    load_cell(cs + lane < ce ? clist[cs + lane] : -1);
     for (int64_t e = cs; e < ce; e += 64) {
       wave_sync();
-      tile[0][lane] = na;
+      tile[0][lane] = make_float4((float)(na.x - B.cx), (float)(na.y - B.cy), (float)(na.z - B.cz), (float)na.w);
       tile[1][lane] = nb;
       tile[2][lane] = nc;
       wave_sync();
-      if (e + 64 + lane < ce) {
-        na = clist[e + 64 + lane].a;
-        nb = clist[e + 64 + lane].b;
-        nc = clist[e + 64 + lane].c;
-      }
+      load_cell(v_nxt);
+      v_nxt = e + 128 + lane < ce ? clist[e + 128 + lane] : -1;
       const int cnt = (int)(ce - e < 64 ? ce - e : 64);
       for (int k = 0; k < cnt; k++) {
         const float4 a = tile[0][k], b = tile[1][k], c = tile[2][k];
@@ -646,7 +603,6 @@ __device__ __forceinline__ void eval_item(
   const int64_t ks = k0 + CPT * ka0;
   const int64_t ke = k0 + CPT * kb0 < k1 ? k0 + CPT * kb0 : k1;
   if (kb0 > 0 && ks < ke) {
-    const float4* kl4 = reinterpret_cast<const float4*>(klist);  // ChunkEnt = 2 x 16 B
     float4* went = tile[1];                                    // (shift x, y, z, first source bits)
     int* wpre = reinterpret_cast<int*>(tile[2]);                // [0..64]
     int64_t w = ks;  // first chunk of the window
@@ -658,8 +614,9 @@ __device__ __forceinline__ void eval_item(
       float4 ex = make_float4(0.f, 0.f, 0.f, 0.f);
       int cnt = 0;
       if (e < ke) {
-        ex = kl4[2 * e];
-        cnt = __float_as_int(kl4[2 * e + 1].x);
+        const NodeD& r = nodes[klist[e]];
+        ex = make_float4((float)(r.cx - B.cx), (float)(r.cy - B.cy), (float)(r.cz - B.cz), __int_as_float(r.start));
+        cnt = r.count;
       }
       int inc = cnt;
 #pragma unroll
@@ -764,6 +721,7 @@ template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
 __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     int64_t g0, int64_t ngroups, const int32_t* __restrict__ ioff, int64_t m, int gs,
     const double4* __restrict__ tpos, const float4* __restrict__ src32, const int32_t* __restrict__ pleaf,
+    const NodeD* __restrict__ nodes, const float4* __restrict__ quad32,
     const double* __restrict__ com, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
     const ChunkEnt* __restrict__ klist, float eps2, const int32_t* __restrict__ gt0,
@@ -791,7 +749,8 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     const int64_t tiles_g = (coff[g + 1] - coff[g] + 63) / 64 + (koff[g + 1] - koff[g] + CPT - 1) / CPT;
     const int64_t ta = j * ITEM_TILES;
     const int64_t tb = ta + ITEM_TILES < tiles_g ? ta + ITEM_TILES : tiles_g;
-    eval_item<TPL, CH, ZEPS, SELF, KAHAN, QUAD>(g, ta, tb, lane, tiles[w], m, gs, tpos, src32, pleaf, com, dim,
+    eval_item<TPL, CH, ZEPS, SELF, KAHAN, QUAD>(g, ta, tb, lane, tiles[w], m, gs, tpos, src32, pleaf, nodes,
+                                                quad32, com, dim,
                                                 boxes, coff, clist, koff, klist, eps2, gt0,
                                                 part + item * (int64_t)(64 * TPL * 3));
   }
@@ -906,6 +865,56 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
   }
 }
 
+// Fill order on the device (the planned path, which has no host scan): a
+// counting sort of the groups by list length, longest first, in
+// ORD_BUCKETS buckets of 4 entries (longer lists share the first bucket).
+// The order within a bucket depends on atomics; the lists do not depend on
+// the order at all.
+constexpr int ORD_BUCKETS = 4096;
+
+__device__ __forceinline__ int order_bucket(int64_t len) {
+  // (counts left unwritten by a walk that overflowed its stack fall in a
+  // bucket too; that walk's lists are discarded)
+  const int64_t q = len > 0 ? len >> 2 : 0;
+  return ORD_BUCKETS - 1 - (int)(q < ORD_BUCKETS - 1 ? q : ORD_BUCKETS - 1);
+}
+
+__global__ __launch_bounds__(256) void k_order_hist(int64_t ng, const int64_t* __restrict__ cnt,
+                                                    uint32_t* __restrict__ hist) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  atomicAdd(&hist[order_bucket(cnt[g] + cnt[ng + g])], 1u);
+}
+
+// exclusive scan of the ORD_BUCKETS counts in place: one wavefront, 64
+// consecutive buckets per lane
+__global__ __launch_bounds__(64) void k_order_scan(uint32_t* __restrict__ hist) {
+  constexpr int PER = ORD_BUCKETS / 64;
+  const int lane = threadIdx.x;
+  uint32_t s = 0;
+  for (int i = 0; i < PER; i++) s += hist[lane * PER + i];
+  uint32_t inc = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off);
+    if (lane >= off) inc += y;
+  }
+  uint32_t run = inc - s;
+  for (int i = 0; i < PER; i++) {
+    const uint32_t v = hist[lane * PER + i];
+    hist[lane * PER + i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_order_scatter(int64_t ng, const int64_t* __restrict__ cnt,
+                                                       uint32_t* __restrict__ cursor, int32_t* __restrict__ ord) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const uint32_t p = atomicAdd(&cursor[order_bucket(cnt[g] + cnt[ng + g])], 1u);
+  ord[p] = (int32_t)g;
+}
+
 // ---------------------------------------------------------------- host
 // Bytes of interaction lists materialised at once by the host-scanned path
 // unless the context sets another budget: a quarter of the device memory, at
@@ -949,7 +958,8 @@ static void launch_eval(bh_ctx* c, int64_t g0, int64_t ng, const int32_t* ioff, 
     const void* kern = reinterpret_cast<const void*>(&k_group_eval<TPL, CH, Z, S, K, Q>);                         \
     hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3(eval_grid(c, kern, items_bound)),                 \
                        dim3(64 * EV_WAVES), 0, c->stream, g0, ng, ioff, m, gs, tpos, t->spart32, t->pleaf,        \
-                       t->com, t->dim, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part);            \
+                       t->nodes, t->quad32, t->com, t->dim, boxes, coff, clist, koff, klist, eps2, gt0, err,  \
+                       next, part);                                                                           \
   }
 #define BH_EV_Q(Z, S, K) \
   if (quad) BH_EV(Z, S, K, true) else BH_EV(Z, S, K, false)
@@ -1074,7 +1084,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / per_wave));
     lds = (size_t)wpb * per_wave;
     ProfScope p(c, "k_group_walk_count");
-    launch_walk<false>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, (int)quad, boxes, itheta2, stack_cap, ch, cnt,
+    launch_walk<false>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, boxes, itheta2, stack_cap, cnt,
                        cnt + ng, cnt + 2 * ng, nullptr, nullptr, nullptr, nullptr, err, groot, 0, ng, wpb, tpos, m,
                        gt0, nullptr);
     BH_LAUNCH_CHECK();
@@ -1095,6 +1105,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   std::vector<int64_t> ho, hc;
   CellEnt* clist;
   ChunkEnt* klist;
+  int32_t* dord = nullptr;  // the planned path's fill order (device-sorted)
   int32_t* ioff = s.get<int32_t>(ng + 1);
   if (!ioff) return BH_ENOMEM;
   int64_t cell_int = 0, pair_int = 0, ntc = 0, ntk = 0, max_items = 0;
@@ -1108,6 +1119,19 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     clist = s.get<CellEnt>((size_t)std::max<int64_t>(plan->cap_cells, 1));
     klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(plan->cap_chunks, 1));
     batches.push_back({0, ng, max_items});
+    // fill order, longest lists first
+    dord = s.get<int32_t>((size_t)ng);
+    uint32_t* hist = s.get<uint32_t>(ORD_BUCKETS);
+    if (!dord || !hist) return BH_ENOMEM;
+    {
+      int rc = fill_u32(c, hist, 0u, ORD_BUCKETS);
+      if (rc) return rc;
+    }
+    const unsigned ob = (unsigned)cdiv(ng, 256);
+    hipLaunchKernelGGL(k_order_hist, dim3(ob), dim3(256), 0, c->stream, ng, cnt, hist);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, c->stream, hist);
+    hipLaunchKernelGGL(k_order_scatter, dim3(ob), dim3(256), 0, c->stream, ng, cnt, hist, dord);
+    BH_LAUNCH_CHECK();
   } else {
     hc.resize(3 * ng);
     for (;;) {
@@ -1173,7 +1197,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const bool self = tgt == nullptr;
   std::vector<int64_t> rel;
   std::vector<int32_t> reli, hord, bucket;
-  int32_t* ord = nullptr;
+  int32_t* ord = dord;
   if (!plan) {
     int64_t maxnb = 0;
     for (const Batch& b : batches) maxnb = std::max(maxnb, b.g1 - b.g0);
@@ -1219,7 +1243,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     }
     {
       ProfScope p(c, "k_group_walk_fill");
-      launch_walk<true>(c, t, dim3((unsigned)cdiv(nb, wpb)), lds, gs, (int)quad, boxes, itheta2, stack_cap, ch,
+      launch_walk<true>(c, t, dim3((unsigned)cdiv(nb, wpb)), lds, gs, boxes, itheta2, stack_cap,
                         nullptr, nullptr, nullptr, offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos,
                         m, gt0, ord);
       BH_LAUNCH_CHECK();
