@@ -96,6 +96,9 @@ struct bh_ctx {
   bool capturing = false;
   // bytes of FP32 interaction lists materialised at once (0: default)
   size_t list_budget = 0;
+  // entries per group of the FP32 one-pass list regions: the longest list of
+  // the previous host-scanned call, with a margin
+  int64_t list_hint = 8192;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   struct ProfRec {

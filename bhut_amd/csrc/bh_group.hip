@@ -13,7 +13,7 @@
 //     centre of mass (exact test: the lanes hold the targets and measure one
 //     candidate node at a time).  Either way every target of the group would
 //     also accept the node under bhut's own test (bh.py:586-601).  Lanes are compacted
-//     with ballots into the cell list, the leaf-chunk list and the stack.
+//     with ballots into the cell list, the leaf list and the stack.
 //     Run twice: count, then fill.
 //  3. k_group_eval: persistent wavefronts over work items (a group's list
 //     cut into pieces of at most 64 tiles), one (or two) targets per lane.
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
                                                    int64_t g0, int64_t nlaunch, int wpb,
                                                    const double4* __restrict__ tpos, int64_t m,
                                                    const int32_t* __restrict__ gt0,
-                                                   const int32_t* __restrict__ order) {
+                                                   const int32_t* __restrict__ order, int64_t cap) {
   extern __shared__ int32_t lds_stacks[];
   // lists of fixed capacity that would overflow are not written at all
   if (FILL && (__builtin_amdgcn_readfirstlane(*err) & 2u)) return;
@@ -293,8 +293,12 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
   __builtin_amdgcn_s_waitcnt(0);
   int sp = 1;
   int64_t nc = 0, nk = 0, np = 0;
-  const int64_t cb = FILL ? coff[g] : 0, ce = FILL ? coff[g + 1] : 0;
-  const int64_t kb = FILL ? koff[g] : 0, ke = FILL ? koff[g + 1] : 0;
+  // list ranges: from the scanned offsets, or (cap > 0) fixed regions of
+  // cap entries per group, with the counts written too
+  const int64_t cb = !FILL ? 0 : cap ? (g - g0) * cap : coff[g];
+  const int64_t ce = !FILL ? 0 : cap ? cb + cap : coff[g + 1];
+  const int64_t kb = !FILL ? 0 : cap ? (g - g0) * cap : koff[g];
+  const int64_t ke = !FILL ? 0 : cap ? kb + cap : koff[g + 1];
   const uint64_t lt = lanemask_lt();
   while (sp > 0) {
     const int k = sp < 64 ? sp : 64;
@@ -374,7 +378,8 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
         const int64_t o = kb + nk + kx;
         if (o < ke) klist[o] = v;
       }
-    } else if (kind == 2) {
+    }
+    if ((!FILL || cap) && kind == 2) {
       np += nd.count;  // this lane's share; summed over the lanes at the end
     }
     nc += __popcll(bc);
@@ -387,7 +392,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
     sp += push;
     wave_sync();
   }
-  if (!FILL) {
+  if (!FILL || cap) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) np += __shfl_xor(np, off);
     if (lane == 0) {
@@ -405,11 +410,11 @@ void launch_walk(bh_ctx* c, const bh_tree* t, dim3 grid, size_t lds, int gs, con
                  double itheta2, int stack_cap, int64_t* ccount, int64_t* kcount, int64_t* pcount,
                  const int64_t* coff, const int64_t* koff, CellEnt* clist, ChunkEnt* klist, uint32_t* err,
                  const int32_t* groot, int64_t g0, int64_t nlaunch, int wpb, const double4* tpos, int64_t m,
-                 const int32_t* gt0, const int32_t* order) {
+                 const int32_t* gt0, const int32_t* order, int64_t cap = 0) {
 #define BH_WALK(D, TPW)                                                                                        \
   hipLaunchKernelGGL((k_group_walk<D, FILL, TPW>), grid, dim3(64 * WALK_WAVES), lds, c->stream, t->nodes,     \
                      boxes, itheta2, stack_cap, ccount, kcount, pcount, coff,                       \
-                     koff, clist, klist, err, groot, g0, nlaunch, wpb, tpos, m, gt0, order)
+                     koff, clist, klist, err, groot, g0, nlaunch, wpb, tpos, m, gt0, order, cap)
   if (t->dim == 3) {
     if (gs == 64) BH_WALK(3, 1);
     else BH_WALK(3, 2);
@@ -521,6 +526,7 @@ __device__ __forceinline__ void eval_item(
     const NodeD* __restrict__ nodes, const float4* __restrict__ quad32,
     const double* __restrict__ com, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
+    const int64_t* __restrict__ ccnt, const int64_t* __restrict__ kcnt,
     const ChunkEnt* __restrict__ klist, float eps2, const int32_t* __restrict__ gt0, double* __restrict__ part) {
   constexpr int CPT = 64 / CH;
   int64_t t0, t1;
@@ -548,7 +554,7 @@ __device__ __forceinline__ void eval_item(
     }
     sx[q] = sy[q] = sz[q] = cx[q] = cy[q] = cz[q] = px[q] = py[q] = pz[q] = 0.f;
   }
-  const int64_t c0 = coff[g], c1 = coff[g + 1];
+  const int64_t c0 = coff[g], c1 = c0 + ccnt[g];
   const int64_t ctiles = (c1 - c0 + 63) / 64;
 
   // ---- accepted cells of tiles [ta, min(tb, ctiles)): each lane gathers
@@ -597,7 +603,7 @@ __device__ __forceinline__ void eval_item(
   // LDS tiles the cell part used: entries (shift xyz, first source) in
   // tile[1], the exclusive prefix of their counts in tile[2]; each lane finds
   // the chunk of its stream position by a binary search of that prefix.
-  const int64_t k0 = koff[g], k1 = koff[g + 1];
+  const int64_t k0 = koff[g], k1 = k0 + kcnt[g];
   const int64_t ka0 = ta > ctiles ? ta - ctiles : 0;
   const int64_t kb0 = tb - ctiles;
   const int64_t ks = k0 + CPT * ka0;
@@ -724,6 +730,7 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     const NodeD* __restrict__ nodes, const float4* __restrict__ quad32,
     const double* __restrict__ com, int dim, const GroupBox* __restrict__ boxes,
     const int64_t* __restrict__ coff, const CellEnt* __restrict__ clist, const int64_t* __restrict__ koff,
+    const int64_t* __restrict__ ccnt, const int64_t* __restrict__ kcnt,
     const ChunkEnt* __restrict__ klist, float eps2, const int32_t* __restrict__ gt0,
     const uint32_t* __restrict__ err, uint32_t* __restrict__ next, double* __restrict__ part) {
   constexpr int CPT = 64 / CH;
@@ -746,12 +753,12 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     }
     const int64_t g = g0 + lo;
     const int64_t j = item - ioff[lo];
-    const int64_t tiles_g = (coff[g + 1] - coff[g] + 63) / 64 + (koff[g + 1] - koff[g] + CPT - 1) / CPT;
+    const int64_t tiles_g = (ccnt[g] + 63) / 64 + (kcnt[g] + CPT - 1) / CPT;
     const int64_t ta = j * ITEM_TILES;
     const int64_t tb = ta + ITEM_TILES < tiles_g ? ta + ITEM_TILES : tiles_g;
     eval_item<TPL, CH, ZEPS, SELF, KAHAN, QUAD>(g, ta, tb, lane, tiles[w], m, gs, tpos, src32, pleaf, nodes,
                                                 quad32, com, dim,
-                                                boxes, coff, clist, koff, klist, eps2, gt0,
+                                                boxes, coff, clist, koff, ccnt, kcnt, klist, eps2, gt0,
                                                 part + item * (int64_t)(64 * TPL * 3));
   }
 }
@@ -919,6 +926,8 @@ __global__ __launch_bounds__(256) void k_order_scatter(int64_t ng, const int64_t
 // Bytes of interaction lists materialised at once by the host-scanned path
 // unless the context sets another budget: a quarter of the device memory, at
 // most 64 GiB (12 GiB when the device memory is unknown).
+constexpr int64_t LIST_HINT_MAX = 65536;  // one-pass list region, entries per group at most
+
 static size_t default_list_budget(const bh_ctx* c) {
   if (c->mem_total == 0) return size_t(12) << 30;
   return std::min<size_t>(size_t(64) << 30, c->mem_total / 4);
@@ -951,15 +960,15 @@ template <int TPL, int CH>
 static void launch_eval(bh_ctx* c, int64_t g0, int64_t ng, const int32_t* ioff, bool zeps, bool self, bool kahan,
                         bool quad, int64_t m, int gs, const double4* tpos, const bh_tree* t, const GroupBox* boxes,
                         const int64_t* coff, const CellEnt* clist, const int64_t* koff, const ChunkEnt* klist,
-                        float eps2, const int32_t* gt0, const uint32_t* err, uint32_t* next, double* part,
+                        const int64_t* ccnt, const int64_t* kcnt, float eps2, const int32_t* gt0, const uint32_t* err, uint32_t* next, double* part,
                         int64_t items_bound) {
 #define BH_EV(Z, S, K, Q)                                                                                          \
   {                                                                                                                \
     const void* kern = reinterpret_cast<const void*>(&k_group_eval<TPL, CH, Z, S, K, Q>);                         \
     hipLaunchKernelGGL((k_group_eval<TPL, CH, Z, S, K, Q>), dim3(eval_grid(c, kern, items_bound)),                 \
                        dim3(64 * EV_WAVES), 0, c->stream, g0, ng, ioff, m, gs, tpos, t->spart32, t->pleaf,        \
-                       t->nodes, t->quad32, t->com, t->dim, boxes, coff, clist, koff, klist, eps2, gt0, err,  \
-                       next, part);                                                                           \
+                       t->nodes, t->quad32, t->com, t->dim, boxes, coff, clist, koff, ccnt, kcnt, klist, eps2, \
+                       gt0, err, next, part);                                                                 \
   }
 #define BH_EV_Q(Z, S, K) \
   if (quad) BH_EV(Z, S, K, true) else BH_EV(Z, S, K, false)
@@ -1077,19 +1086,45 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   }
   int wpb = 1;
   size_t lds = 0;
-  // the count pass; an LDS stack overflow on the host-scanned path reruns it
-  // with twice the stack (the lists do not depend on the stack size)
+  // One pass when it fits: the host-scanned path writes the lists straight
+  // into regions of dcap entries per group, counting them at the same time,
+  // if those regions fit the list budget (dcap: the context's longest list of
+  // its previous call, with a margin).  Otherwise -- or when a list outgrows
+  // its region -- the groups are counted first and filled at scanned offsets.
+  int64_t dcap = 0;
+  CellEnt* dclist = nullptr;
+  ChunkEnt* dklist = nullptr;
+  if (!plan) {
+    const size_t budget = c->list_budget ? c->list_budget : default_list_budget(c);
+    const int64_t want = c->list_hint;
+    if ((size_t)ng * (size_t)want * 2 * sizeof(int32_t) <= budget) {
+      dclist = s.get<CellEnt>((size_t)(ng * want));
+      dklist = s.get<ChunkEnt>((size_t)(ng * want));
+      if (dclist && dklist) dcap = want;
+      (void)hipGetLastError();
+    }
+  }
+  // the count (or one-pass) walk; an LDS stack overflow on the host-scanned
+  // path reruns it with twice the stack (the lists do not depend on it)
   auto count_pass = [&]() -> int {
     const int64_t per_wave = (int64_t)stack_cap * 4 + CBUF * (int64_t)sizeof(double4);
     wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / per_wave));
     lds = (size_t)wpb * per_wave;
-    ProfScope p(c, "k_group_walk_count");
-    launch_walk<false>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, boxes, itheta2, stack_cap, cnt,
-                       cnt + ng, cnt + 2 * ng, nullptr, nullptr, nullptr, nullptr, err, groot, 0, ng, wpb, tpos, m,
-                       gt0, nullptr);
+    if (dcap) {
+      ProfScope p(c, "k_group_walk_fill");
+      launch_walk<true>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, boxes, itheta2, stack_cap, cnt, cnt + ng,
+                        cnt + 2 * ng, nullptr, nullptr, dclist, dklist, err, groot, 0, ng, wpb, tpos, m, gt0,
+                        nullptr, dcap);
+    } else {
+      ProfScope p(c, "k_group_walk_count");
+      launch_walk<false>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, boxes, itheta2, stack_cap, cnt,
+                         cnt + ng, cnt + 2 * ng, nullptr, nullptr, nullptr, nullptr, err, groot, 0, ng, wpb, tpos,
+                         m, gt0, nullptr);
+    }
     BH_LAUNCH_CHECK();
     return BH_OK;
   };
+  bool direct = false;  // the lists are in place after the one pass
   {
     int rc = count_pass();
     if (rc) return rc;
@@ -1151,18 +1186,25 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       if (rc) return rc;
     }
     const size_t budget = c->list_budget ? c->list_budget : default_list_budget(c);
+    int64_t longest = 0;
+    for (int64_t g = 0; g < ng; g++) longest = std::max(longest, std::max(hc[g], hc[ng + g]));
+    direct = dcap > 0 && longest <= dcap;
+    // regions for the next call: the longest list and a quarter more, in
+    // multiples of 1024 entries
+    c->list_hint = std::min<int64_t>(LIST_HINT_MAX, std::max<int64_t>(1024, (longest + longest / 4 + 1023) & ~int64_t(1023)));
     ho.assign(2 * (ng + 1), 0);
     int64_t maxc = 0, maxk = 0, bc = 0, bk = 0, bi = 0;
     size_t bytes = 0;
     batches.push_back({0, 0, 0});
     for (int64_t g = 0; g < ng; g++) {
-      ho[g + 1] = ho[g] + hc[g];
-      ho[ng + 1 + g + 1] = ho[ng + 1 + g] + hc[ng + g];
+      // one-pass lists start at g * dcap; scanned lists are packed
+      ho[g + 1] = direct ? (g + 1) * dcap : ho[g] + hc[g];
+      ho[ng + 1 + g + 1] = direct ? (g + 1) * dcap : ho[ng + 1 + g] + hc[ng + g];
       const int64_t tg = gt0 ? t->h_gtab[g + 1] - t->h_gtab[g] : std::min<int64_t>(gs, m - g * gs);
       cell_int += hc[g] * tg;
       pair_int += hc[2 * ng + g] * tg;
       const size_t gb = hc[g] * sizeof(CellEnt) + hc[ng + g] * sizeof(ChunkEnt);
-      if (g > batches.back().g0 && bytes + gb > budget) {
+      if (!direct && g > batches.back().g0 && bytes + gb > budget) {
         batches.back().g1 = g;
         batches.back().items = bi;
         batches.push_back({g, 0, 0});
@@ -1182,10 +1224,17 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     maxk = std::max(maxk, bk);
     for (const Batch& b : batches) max_items = std::max(max_items, b.items);
     if (!tgt) pair_int -= m;  // each target's own pair contributes nothing
-    ntc = ho[ng];
-    ntk = ho[2 * ng + 1];
-    clist = s.get<CellEnt>((size_t)std::max<int64_t>(maxc, 1));
-    klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(maxk, 1));
+    for (int64_t g = 0; g < ng; g++) {
+      ntc += hc[g];
+      ntk += hc[ng + g];
+    }
+    if (direct || (dcap && maxc <= ng * dcap && maxk <= ng * dcap)) {
+      clist = dclist;  // the one-pass regions (in place, or reused for the packed lists)
+      klist = dklist;
+    } else {
+      clist = s.get<CellEnt>((size_t)std::max<int64_t>(maxc, 1));
+      klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(maxk, 1));
+    }
   }
   if (!clist || !klist) return BH_ENOMEM;
   const int tpg = gs;  // targets per group = 64 * TPL
@@ -1206,7 +1255,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   }
   for (const Batch& b : batches) {
     const int64_t nb = b.g1 - b.g0;
-    if (!plan) {
+    if (!plan && !direct) {
       // fill order: groups with the longest lists (the longest walks) first,
       // so the pass does not end on a few long walks started late; a
       // counting sort on the list length, in 4096 buckets
@@ -1224,6 +1273,8 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       hord.resize(nb);
       for (int64_t g = b.g0; g < b.g1; g++) hord[bucket[4095 - ((hc[g] + hc[ng + g]) >> sh)]++] = (int32_t)(g - b.g0);
       BH_HIP(hipMemcpyAsync(ord, hord.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (!plan) {
       // offsets of this batch's groups and items, relative to its first
       // list entries and first item
       rel.resize(2 * (nb + 1));
@@ -1241,7 +1292,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       BH_HIP(hipMemcpyAsync(ioff, reli.data(), (nb + 1) * 4, hipMemcpyHostToDevice, c->stream));
       BH_HIP(hipStreamSynchronize(c->stream));  // rel, reli, hord are reused by the next batch
     }
-    {
+    if (!direct) {
       ProfScope p(c, "k_group_walk_fill");
       launch_walk<true>(c, t, dim3((unsigned)cdiv(nb, wpb)), lds, gs, boxes, itheta2, stack_cap,
                         nullptr, nullptr, nullptr, offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos,
@@ -1255,13 +1306,13 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       const int64_t* coff = offs;
       const int64_t* koff = offs + ng + 1;
       if (gs == 64) {
-        if (ch == 16) launch_eval<1, 16>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
-        else if (ch == 32) launch_eval<1, 32>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
-        else launch_eval<1, 64>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
+        if (ch == 16) launch_eval<1, 16>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
+        else if (ch == 32) launch_eval<1, 32>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
+        else launch_eval<1, 64>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
       } else {
-        if (ch == 16) launch_eval<2, 16>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
-        else if (ch == 32) launch_eval<2, 32>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
-        else launch_eval<2, 64>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, eps2, gt0, err, next, part, b.items);
+        if (ch == 16) launch_eval<2, 16>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
+        else if (ch == 32) launch_eval<2, 32>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
+        else launch_eval<2, 64>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
       }
       BH_LAUNCH_CHECK();
     }

@@ -122,6 +122,27 @@ def test_api_precision_keyword(native):
         t.accelerations(precision="fp16")
 
 
+def test_one_pass_lists_match_counted_lists(native):
+    """The lists are written in one pass into per-group regions sized by the
+    context's previous call, or -- when a list outgrows its region, or the
+    regions exceed the list budget -- counted first and packed.  At theta = 0
+    every leaf is in every group's list (16384 leaves > the first call's 8192-
+    entry regions), so a fresh context takes the counted path first and the
+    one-pass path next; the lists and so the results are bit-identical."""
+    x, m = workloads.plummer(300_000, seed=13)
+    c = native.Context(0)
+    t = native.DeviceTree.build(x, m, 32, context=c)
+    first = t.accel_f32(None, 0.0, 0.01, 1.0)   # regions too small: counted and packed
+    second = t.accel_f32(None, 0.0, 0.01, 1.0)  # regions sized by the first call: one pass
+    np.testing.assert_array_equal(second.view(np.uint64), first.view(np.uint64))
+    c.set_list_budget(1 << 20)                   # regions over budget: counted, in batches
+    third = t.accel_f32(None, 0.0, 0.01, 1.0)
+    np.testing.assert_array_equal(third.view(np.uint64), first.view(np.uint64))
+    d = native.direct(x[:2000], m[:2000], softening=0.01)
+    sub = native.DeviceTree.build(x[:2000], m[:2000], 32, context=c).accel_f32(None, 0.0, 0.01, 1.0)
+    assert rel_err(sub, d)[0] < 2e-6
+
+
 def test_list_batches_give_identical_results(native):
     """A small list budget splits the groups into many batches (each filled,
     ordered and evaluated on its own); every group's lists and sums are the
