@@ -425,7 +425,7 @@ class Simulation:
     alive while this object lives)."""
 
     INFO_KEYS = ("steps", "rebuilds", "refits", "graph_launches", "graph_captures", "list_regrows",
-                 "cell_capacity", "chunk_capacity")
+                 "list_region", "item_capacity")
 
     def __init__(self, pos_ptr: int, vel_ptr: int, mass_ptr: int, n: int, dim: int, *, leaf_size: int = 32,
                  theta: float = 0.5, softening: float = 0.0, G: float = 1.0, precision: int = 32,
@@ -443,8 +443,10 @@ class Simulation:
     def accelerations_dev(self, out_ptr: int) -> None:
         check(load().bh_sim_accelerations(self.handle, vp(out_ptr)))
 
-    def set_list_capacity(self, cells: int, chunks: int) -> None:
-        check(load().bh_sim_set_list_capacity(self.handle, int(cells), int(chunks)))
+    def set_list_capacity(self, region: int, items: int) -> None:
+        """FP32 list capacities of captured steps: entries per group of each
+        list region, and work items."""
+        check(load().bh_sim_set_list_capacity(self.handle, int(region), int(items)))
 
     def info(self) -> dict:
         out = np.zeros(8, np.int64)

@@ -240,8 +240,11 @@ int f32_prepare(bh_ctx* c, bh_tree* t);
 // synchronisation.  When a list would exceed its capacity, bit 1 of *err is
 // set, the lists and the evaluation are skipped, and need[0..1] receive the
 // required capacities.  *err must be zero on entry.
+// A planned FP32 walk (no host synchronisation, e.g. under graph capture):
+// lists in regions of cap_region entries per group, at most cap_items work
+// items; an overflow sets bit 2 of *err and need[] = (longest list, items).
 struct F32Plan {
-  int64_t cap_cells = 0, cap_chunks = 0;
+  int64_t cap_region = 0, cap_items = 0;
   uint32_t* err = nullptr;  // device
   int64_t* need = nullptr;  // device, 2 entries
 };

@@ -790,136 +790,75 @@ __global__ __launch_bounds__(256) void k_item_reduce(int64_t g0, int64_t ngroups
   if (dim == 3) out[o * dim + 2] = G * a2;
 }
 
-// Exclusive scans of the per-group cell and chunk counts into list offsets
-// and of the per-group item counts into item offsets, on the device (one
-// workgroup), with the capacity check of the fixed-size lists and partial
-// buffers: totals above capacity set bit 1 of *err.
+// The planned path's list layout and work items, on the device: each
+// group's lists sit in regions of cap entries (coff[g] = koff[g] = g * cap),
+// ioff[] is the prefix of the groups' item counts.  A list longer than its
+// region, or more items than cap_items, sets bit 2 of *err (the evaluation
+// then does nothing); need[0] = the longest list, need[1] = the items.
 __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* __restrict__ cnt, int cpt,
                                                      int64_t* __restrict__ offs, int32_t* __restrict__ ioff,
-                                                     int64_t cap_c, int64_t cap_k, int64_t cap_items,
-                                                     uint32_t* __restrict__ err, int64_t* __restrict__ need) {
+                                                     int64_t cap, int64_t cap_items, uint32_t* __restrict__ err,
+                                                     int64_t* __restrict__ need) {
   constexpr int IT = 8;
-  __shared__ int64_t wsum[3][16];
+  __shared__ int64_t wsum[16], wmax[16];
   const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
   int64_t* coff = offs;
   int64_t* koff = offs + ng + 1;
-  int64_t carry_c = 0, carry_k = 0, carry_i = 0;
+  int64_t carry_i = 0, mx = 0;
   for (int64_t base = 0; base < ng; base += 1024 * IT) {
-    int64_t vc[IT], vk[IT], vi[IT];
-    int64_t sc = 0, sk = 0, si = 0;
+    int64_t vi[IT];
+    int64_t si = 0;
 #pragma unroll
     for (int i = 0; i < IT; i++) {
       const int64_t g = base + (int64_t)tid * IT + i;
-      vc[i] = g < ng ? cnt[g] : 0;
-      vk[i] = g < ng ? cnt[ng + g] : 0;
-      vi[i] = g < ng ? group_items(vc[i], vk[i], cpt) : 0;
-      sc += vc[i];
-      sk += vk[i];
+      const int64_t vc = g < ng ? cnt[g] : 0, vk = g < ng ? cnt[ng + g] : 0;
+      vi[i] = g < ng ? group_items(vc, vk, cpt) : 0;
       si += vi[i];
+      mx = vc > mx ? vc : mx;
+      mx = vk > mx ? vk : mx;
     }
-    int64_t ic = sc, ik = sk, ii = si;  // inclusive wave scan of the thread sums
+    int64_t ii = si;  // inclusive wave scan of the thread sums
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-      const int64_t yc = __shfl_up(ic, off), yk = __shfl_up(ik, off), yi = __shfl_up(ii, off);
-      if (lane >= off) {
-        ic += yc;
-        ik += yk;
-        ii += yi;
-      }
+      const int64_t yi = __shfl_up(ii, off);
+      if (lane >= off) ii += yi;
     }
-    if (lane == 63) {
-      wsum[0][w] = ic;
-      wsum[1][w] = ik;
-      wsum[2][w] = ii;
-    }
+    if (lane == 63) wsum[w] = ii;
     __syncthreads();
-    int64_t pc = carry_c + ic - sc, pk = carry_k + ik - sk, pi = carry_i + ii - si;
-    int64_t tc = 0, tk = 0, tn = 0;
+    int64_t pi = carry_i + ii - si, tn = 0;
     for (int j = 0; j < 16; j++) {
-      if (j < w) {
-        pc += wsum[0][j];
-        pk += wsum[1][j];
-        pi += wsum[2][j];
-      }
-      tc += wsum[0][j];
-      tk += wsum[1][j];
-      tn += wsum[2][j];
+      if (j < w) pi += wsum[j];
+      tn += wsum[j];
     }
     __syncthreads();  // wsum is rewritten by the next chunk
 #pragma unroll
     for (int i = 0; i < IT; i++) {
       const int64_t g = base + (int64_t)tid * IT + i;
       if (g < ng) {
-        coff[g] = pc;
-        koff[g] = pk;
+        coff[g] = g * cap;
+        koff[g] = g * cap;
         ioff[g] = (int32_t)pi;
       }
-      pc += vc[i];
-      pk += vk[i];
       pi += vi[i];
     }
-    carry_c += tc;
-    carry_k += tk;
     carry_i += tn;
   }
-  if (tid == 0) {
-    coff[ng] = carry_c;
-    koff[ng] = carry_k;
-    ioff[ng] = (int32_t)carry_i;
-    need[0] = carry_c;
-    need[1] = carry_k;
-    if (carry_c > cap_c || carry_k > cap_k || carry_i > cap_items) atomicOr(err, 2u);
-  }
-}
-
-// Fill order on the device (the planned path, which has no host scan): a
-// counting sort of the groups by list length, longest first, in
-// ORD_BUCKETS buckets of 4 entries (longer lists share the first bucket).
-// The order within a bucket depends on atomics; the lists do not depend on
-// the order at all.
-constexpr int ORD_BUCKETS = 4096;
-
-__device__ __forceinline__ int order_bucket(int64_t len) {
-  // (counts left unwritten by a walk that overflowed its stack fall in a
-  // bucket too; that walk's lists are discarded)
-  const int64_t q = len > 0 ? len >> 2 : 0;
-  return ORD_BUCKETS - 1 - (int)(q < ORD_BUCKETS - 1 ? q : ORD_BUCKETS - 1);
-}
-
-__global__ __launch_bounds__(256) void k_order_hist(int64_t ng, const int64_t* __restrict__ cnt,
-                                                    uint32_t* __restrict__ hist) {
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= ng) return;
-  atomicAdd(&hist[order_bucket(cnt[g] + cnt[ng + g])], 1u);
-}
-
-// exclusive scan of the ORD_BUCKETS counts in place: one wavefront, 64
-// consecutive buckets per lane
-__global__ __launch_bounds__(64) void k_order_scan(uint32_t* __restrict__ hist) {
-  constexpr int PER = ORD_BUCKETS / 64;
-  const int lane = threadIdx.x;
-  uint32_t s = 0;
-  for (int i = 0; i < PER; i++) s += hist[lane * PER + i];
-  uint32_t inc = s;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(inc, off);
-    if (lane >= off) inc += y;
+  for (int off = 32; off > 0; off >>= 1) {
+    const int64_t y = __shfl_xor(mx, off);
+    mx = y > mx ? y : mx;
   }
-  uint32_t run = inc - s;
-  for (int i = 0; i < PER; i++) {
-    const uint32_t v = hist[lane * PER + i];
-    hist[lane * PER + i] = run;
-    run += v;
+  if (lane == 0) wmax[w] = mx;
+  __syncthreads();
+  if (tid == 0) {
+    for (int j = 0; j < 16; j++) mx = wmax[j] > mx ? wmax[j] : mx;
+    coff[ng] = ng * cap;
+    koff[ng] = ng * cap;
+    ioff[ng] = (int32_t)carry_i;
+    need[0] = mx;
+    need[1] = carry_i;
+    if (mx > cap || carry_i > cap_items) atomicOr(err, 2u);
   }
-}
-
-__global__ __launch_bounds__(256) void k_order_scatter(int64_t ng, const int64_t* __restrict__ cnt,
-                                                       uint32_t* __restrict__ cursor, int32_t* __restrict__ ord) {
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= ng) return;
-  const uint32_t p = atomicAdd(&cursor[order_bucket(cnt[g] + cnt[ng + g])], 1u);
-  ord[p] = (int32_t)g;
 }
 
 // ---------------------------------------------------------------- host
@@ -1094,7 +1033,14 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   int64_t dcap = 0;
   CellEnt* dclist = nullptr;
   ChunkEnt* dklist = nullptr;
-  if (!plan) {
+  if (plan) {
+    // the planned path (no host synchronisation) always takes one pass, into
+    // regions of the planned size
+    dcap = std::max<int64_t>(plan->cap_region, 1);
+    dclist = s.get<CellEnt>((size_t)(ng * dcap));
+    dklist = s.get<ChunkEnt>((size_t)(ng * dcap));
+    if (!dclist || !dklist) return BH_ENOMEM;
+  } else {
     const size_t budget = c->list_budget ? c->list_budget : default_list_budget(c);
     const int64_t want = c->list_hint;
     if ((size_t)ng * (size_t)want * 2 * sizeof(int32_t) <= budget) {
@@ -1140,33 +1086,20 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   std::vector<int64_t> ho, hc;
   CellEnt* clist;
   ChunkEnt* klist;
-  int32_t* dord = nullptr;  // the planned path's fill order (device-sorted)
   int32_t* ioff = s.get<int32_t>(ng + 1);
   if (!ioff) return BH_ENOMEM;
   int64_t cell_int = 0, pair_int = 0, ntc = 0, ntk = 0, max_items = 0;
   if (plan) {
-    // offsets scanned on the device into lists of the planned capacity; the
-    // item count is bounded by the capacities
-    max_items = 2 * ng + (plan->cap_cells / 64 + plan->cap_chunks / cpt + 2 * ng) / ITEM_TILES + 1;
-    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, cpt, offs, ioff, plan->cap_cells,
-                       plan->cap_chunks, max_items, err, plan->need);
+    // the one-pass lists are in place; their layout, the work items and the
+    // capacity checks come from the device
+    max_items = std::max<int64_t>(plan->cap_items, 1);
+    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, cpt, offs, ioff, dcap, max_items,
+                       err, plan->need);
     BH_LAUNCH_CHECK();
-    clist = s.get<CellEnt>((size_t)std::max<int64_t>(plan->cap_cells, 1));
-    klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(plan->cap_chunks, 1));
+    clist = dclist;
+    klist = dklist;
     batches.push_back({0, ng, max_items});
-    // fill order, longest lists first
-    dord = s.get<int32_t>((size_t)ng);
-    uint32_t* hist = s.get<uint32_t>(ORD_BUCKETS);
-    if (!dord || !hist) return BH_ENOMEM;
-    {
-      int rc = fill_u32(c, hist, 0u, ORD_BUCKETS);
-      if (rc) return rc;
-    }
-    const unsigned ob = (unsigned)cdiv(ng, 256);
-    hipLaunchKernelGGL(k_order_hist, dim3(ob), dim3(256), 0, c->stream, ng, cnt, hist);
-    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, c->stream, hist);
-    hipLaunchKernelGGL(k_order_scatter, dim3(ob), dim3(256), 0, c->stream, ng, cnt, hist, dord);
-    BH_LAUNCH_CHECK();
+    direct = true;
   } else {
     hc.resize(3 * ng);
     for (;;) {
@@ -1246,7 +1179,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const bool self = tgt == nullptr;
   std::vector<int64_t> rel;
   std::vector<int32_t> reli, hord, bucket;
-  int32_t* ord = dord;
+  int32_t* ord = nullptr;
   if (!plan) {
     int64_t maxnb = 0;
     for (const Batch& b : batches) maxnb = std::max(maxnb, b.g1 - b.g0);

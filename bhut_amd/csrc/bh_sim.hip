@@ -56,7 +56,7 @@ struct bh_sim {
   hipStream_t stream = nullptr;
   uint32_t* status = nullptr;  // device: status word, then need[2] (int64) at byte 8
   int64_t* need = nullptr;
-  int64_t cap_cells = 0, cap_chunks = 0;
+  int64_t cap_region = 1024, cap_items = 4096;  // FP32 lists of captured steps: entries per group, work items
   // graphs of a refit step (0) and a rebuild step (1)
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -74,8 +74,8 @@ int enqueue_force(bh_sim* s) {
   if (s->precision == 64)
     return walk_mode(c, s->tree, WALK_ACC, nullptr, s->n, s->theta, s->eps, s->G, s->acc, nullptr, nullptr, nullptr);
   F32Plan plan;
-  plan.cap_cells = s->cap_cells;
-  plan.cap_chunks = s->cap_chunks;
+  plan.cap_region = s->cap_region;
+  plan.cap_items = s->cap_items;
   plan.err = s->status;
   plan.need = s->need;
   return walk_f32(c, s->tree, nullptr, s->n, s->theta, s->eps, s->G, &s->opts, s->acc, nullptr, &plan);
@@ -157,9 +157,10 @@ int settle(bh_sim* s, double dt) {
       }
       s->opts.stack_cap = std::min(8192, 2 * s->opts.stack_cap);
     } else {
-      const int64_t nc = (int64_t)h[1], nk = (int64_t)h[2];
-      s->cap_cells = std::max(s->cap_cells, nc + nc / 4 + 4096);
-      s->cap_chunks = std::max(s->cap_chunks, nk + nk / 4 + 4096);
+      // the longest list and the work items, with a quarter to spare
+      const int64_t nl = (int64_t)h[1], ni = (int64_t)h[2];
+      s->cap_region = std::max(s->cap_region, nl + nl / 4 + 1024);
+      s->cap_items = std::max(s->cap_items, ni + ni / 4 + 1024);
     }
     s->regrows++;
     drop_graph(s, 0);
@@ -280,12 +281,12 @@ int bh_sim_accelerations(bh_sim* s, double* d_acc) {
   return BH_OK;
 }
 
-int bh_sim_set_list_capacity(bh_sim* s, int64_t cells, int64_t chunks) {
+int bh_sim_set_list_capacity(bh_sim* s, int64_t region, int64_t items) {
   BH_REQ(s, "simulation is NULL");
-  BH_REQ(cells >= 0 && chunks >= 0, "capacities must be non-negative");
+  BH_REQ(region >= 1 && items >= 1, "capacities must be positive");
   BH_HIP(hipStreamSynchronize(s->stream));
-  s->cap_cells = cells;
-  s->cap_chunks = chunks;
+  s->cap_region = region;
+  s->cap_items = items;
   drop_graph(s, 0);
   drop_graph(s, 1);
   return BH_OK;
@@ -293,8 +294,8 @@ int bh_sim_set_list_capacity(bh_sim* s, int64_t cells, int64_t chunks) {
 
 int bh_sim_info(const bh_sim* s, int64_t* out) {
   BH_REQ(s && out, "NULL argument");
-  const int64_t v[8] = {s->steps, s->rebuilds, s->refits, s->launches, s->captures, s->regrows, s->cap_cells,
-                        s->cap_chunks};
+  const int64_t v[8] = {s->steps, s->rebuilds, s->refits, s->launches, s->captures, s->regrows, s->cap_region,
+                        s->cap_items};
   for (int i = 0; i < 8; i++) out[i] = v[i];
   return BH_OK;
 }

@@ -202,12 +202,13 @@ int bh_sim_create(bh_ctx *ctx, double *d_pos, double *d_vel, const double *d_mas
 int bh_sim_run(bh_sim *sim, double dt, int64_t steps, int32_t rebuild_every, int32_t use_graph);
 /* current accelerations (n, dim) into device memory */
 int bh_sim_accelerations(bh_sim *sim, double *d_acc);
-/* capacities (entries) of the FP32 interaction lists used by captured
- * steps; too small a capacity is detected, regrown, and the step's force
- * redone eagerly.  Drops the captured graphs. */
-int bh_sim_set_list_capacity(bh_sim *sim, int64_t cells, int64_t chunks);
+/* capacities of the FP32 interaction lists used by captured steps: entries
+ * per group of each list region, and work items (both >= 1); too small a
+ * capacity is detected, regrown, and the step's force redone eagerly.  Drops
+ * the captured graphs. */
+int bh_sim_set_list_capacity(bh_sim *sim, int64_t region, int64_t items);
 /* 8 int64: steps, rebuilds, refits, graph launches, graph captures, list
- * regrows, cell capacity, chunk capacity */
+ * regrows, list region (entries per group), item capacity */
 int bh_sim_info(const bh_sim *sim, int64_t *out);
 void bh_sim_destroy(bh_sim *sim);
 

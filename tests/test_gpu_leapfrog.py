@@ -105,7 +105,7 @@ def test_list_overflow_is_regrown(native):
     bits(gx, hx, "positions")
     bits(gv, hv, "velocities")
     assert info["list_regrows"] >= 1
-    assert info["cell_capacity"] > 16 and info["chunk_capacity"] > 16
+    assert info["list_region"] > 16 and info["item_capacity"] > 16
 
 
 def test_group128_and_monopole_options(native):
