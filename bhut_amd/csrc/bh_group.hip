@@ -13,8 +13,9 @@
 //     centre of mass (exact test: the lanes hold the targets and measure one
 //     candidate node at a time).  Either way every target of the group would
 //     also accept the node under bhut's own test (bh.py:586-601).  Lanes are compacted
-//     with ballots into the cell list, the leaf list and the stack.
-//     Run twice: count, then fill.
+//     with ballots into the cell list, the leaf list and the stack.  One
+//     pass writes the lists into per-group regions and counts them; a count
+//     pass and a packed fill are the fallback when a region is too small.
 //  3. k_group_eval: persistent wavefronts over work items (a group's list
 //     cut into pieces of at most 64 tiles), one (or two) targets per lane.
 //     Tiles of 64 cells, or of 64 sources of the item's leaf chunks taken as
