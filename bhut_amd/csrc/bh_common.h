@@ -99,6 +99,8 @@ struct bh_ctx {
   // entries per group of the FP32 one-pass list regions: the longest list of
   // the previous host-scanned call, with a margin
   int64_t list_hint = 8192;
+  // work items of the previous host-scanned FP32 call, with a margin
+  int64_t items_hint = 65536;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   struct ProfRec {

@@ -796,16 +796,20 @@ __global__ __launch_bounds__(256) void k_item_reduce(int64_t g0, int64_t ngroups
 // ioff[] is the prefix of the groups' item counts.  A list longer than its
 // region, or more items than cap_items, sets bit 2 of *err (the evaluation
 // then does nothing); need[0] = the longest list, need[1] = the items.
+// sums (optional): cell and pair interactions (entries x the group's
+// targets), cell and leaf entries.
 __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* __restrict__ cnt, int cpt,
                                                      int64_t* __restrict__ offs, int32_t* __restrict__ ioff,
                                                      int64_t cap, int64_t cap_items, uint32_t* __restrict__ err,
-                                                     int64_t* __restrict__ need) {
+                                                     int64_t* __restrict__ need, int64_t* __restrict__ sums,
+                                                     int64_t m, int gs, const int32_t* __restrict__ gt0) {
   constexpr int IT = 8;
-  __shared__ int64_t wsum[16], wmax[16];
+  __shared__ int64_t wsum[16], wmax[16], wsums[4][16];
   const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
   int64_t* coff = offs;
   int64_t* koff = offs + ng + 1;
   int64_t carry_i = 0, mx = 0;
+  int64_t acc[4] = {0, 0, 0, 0};
   for (int64_t base = 0; base < ng; base += 1024 * IT) {
     int64_t vi[IT];
     int64_t si = 0;
@@ -815,6 +819,14 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
       const int64_t vc = g < ng ? cnt[g] : 0, vk = g < ng ? cnt[ng + g] : 0;
       vi[i] = g < ng ? group_items(vc, vk, cpt) : 0;
       si += vi[i];
+      if (sums && g < ng) {
+        int64_t a, b;
+        group_range(g, m, gs, gt0, a, b);
+        acc[0] += vc * (b - a);
+        acc[1] += cnt[2 * ng + g] * (b - a);
+        acc[2] += vc;
+        acc[3] += vk;
+      }
       mx = vc > mx ? vc : mx;
       mx = vk > mx ? vk : mx;
     }
@@ -849,10 +861,23 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
     const int64_t y = __shfl_xor(mx, off);
     mx = y > mx ? y : mx;
   }
-  if (lane == 0) wmax[w] = mx;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off);
+  if (lane == 0) {
+    wmax[w] = mx;
+    for (int q = 0; q < 4; q++) wsums[q][w] = acc[q];
+  }
   __syncthreads();
   if (tid == 0) {
     for (int j = 0; j < 16; j++) mx = wmax[j] > mx ? wmax[j] : mx;
+    if (sums)
+      for (int q = 0; q < 4; q++) {
+        int64_t t = 0;
+        for (int j = 0; j < 16; j++) t += wsums[q][j];
+        sums[q] = t;
+      }
     coff[ng] = ng * cap;
     koff[ng] = ng * cap;
     ioff[ng] = (int32_t)carry_i;
@@ -1076,10 +1101,96 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     int rc = count_pass();
     if (rc) return rc;
   }
-  // Lists are materialised for batches of consecutive groups whose lists
-  // fit the list budget (one batch unless the problem is very large); each
-  // batch is filled, split into work items, evaluated and reduced in turn.
   const int cpt = 64 / ch;
+  const int tpg = gs;  // targets per group = 64 * TPL
+  const float eps2 = (float)(eps * eps);
+  const bool zeps = !(eps2 > 0.f);
+  const bool self = tgt == nullptr;
+  int32_t* ioff = s.get<int32_t>(ng + 1);
+  uint32_t* next = s.get<uint32_t>(1);
+  if (!ioff || !next) return BH_ENOMEM;
+  // the evaluation and the reduction of groups [g0, g0 + nb): lists at the
+  // offsets in offs, items[ioff], partial sums in part
+  auto evaluate = [&](int64_t g0, int64_t nb, int64_t items, const CellEnt* clist, const ChunkEnt* klist,
+                      double* part) -> int {
+    BH_TRY(fill_u32(c, next, 0u, 1));
+    {
+      ProfScope p(c, "k_group_eval_f32");
+      const int64_t* coff = offs;
+      const int64_t* koff = offs + ng + 1;
+#define BH_EVAL(TPL, CH)                                                                                         \
+  launch_eval<TPL, CH>(c, g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, \
+                       cnt, cnt + ng, eps2, gt0, err, next, part, items)
+      if (gs == 64) {
+        if (ch == 16) BH_EVAL(1, 16);
+        else if (ch == 32) BH_EVAL(1, 32);
+        else BH_EVAL(1, 64);
+      } else {
+        if (ch == 16) BH_EVAL(2, 16);
+        else if (ch == 32) BH_EVAL(2, 32);
+        else BH_EVAL(2, 64);
+      }
+#undef BH_EVAL
+      BH_LAUNCH_CHECK();
+    }
+    {
+      ProfScope p(c, "k_item_reduce");
+      hipLaunchKernelGGL(k_item_reduce, dim3((unsigned)cdiv(nb * tpg, 256)), dim3(256), 0, c->stream, g0, nb, ioff,
+                         m, gs, gt0, part, tpg, torig, t->dim, G, err, out);
+      BH_LAUNCH_CHECK();
+    }
+    return BH_OK;
+  };
+  if (plan || dcap > 0) {
+    // The one-pass lists are in place: their layout, the work items and the
+    // capacity checks come from the device, and the evaluation follows at
+    // once (it does nothing if a check failed).  A planned walk stops here;
+    // the host-scanned path reads the status once at the end and, if a list
+    // outgrew its region or the items their buffer, redoes the lists below.
+    const int64_t cap_items = std::max<int64_t>(plan ? plan->cap_items : c->items_hint, 1);
+    int64_t* need = plan ? plan->need : s.get<int64_t>(6);
+    double* part = s.get<double>((size_t)cap_items * tpg * 3);
+    if (!need || !part) return BH_ENOMEM;
+    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, cpt, offs, ioff, dcap, cap_items,
+                       err, need, plan ? nullptr : need + 2, m, gs, gt0);
+    BH_LAUNCH_CHECK();
+    BH_TRY(evaluate(0, ng, cap_items, dclist, dklist, part));
+    if (plan) return BH_OK;
+    int64_t h[6];
+    uint32_t herr = 0;
+    BH_HIP(hipMemcpyAsync(h, need, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(hipStreamSynchronize(c->stream));
+    if (!(herr & 1)) {
+      // regions and item buffer for the next call: what this one needed and
+      // a quarter more
+      c->list_hint = std::min<int64_t>(LIST_HINT_MAX, std::max<int64_t>(1024, (h[0] + h[0] / 4 + 1023) & ~int64_t(1023)));
+      c->items_hint = std::max<int64_t>(1024, h[1] + h[1] / 4);
+    }
+    if (!herr) {
+      if (stats) {
+        stats[0] = h[2];
+        stats[1] = h[3] - (tgt ? 0 : m);  // each target's own pair contributes nothing
+        stats[2] = ng;
+        stats[3] = h[4] + h[5];
+      }
+      return BH_OK;
+    }
+    BH_TRY(fill_u32(c, err, 0u, 1));
+    if (herr & 1) {
+      if (stack_cap >= MAX_STACK) {
+        set_error("group walk stack overflow (stack_cap=%d)", stack_cap);
+        return BH_EOVERFLOW;
+      }
+      stack_cap = std::min(MAX_STACK, 2 * stack_cap);
+      hint = std::max(hint, stack_cap);
+      BH_TRY(count_pass());
+    }
+  }
+  // Host-scanned lists: materialised for batches of consecutive groups whose
+  // lists fit the list budget (one batch unless the problem is very large);
+  // each batch is filled, split into work items, evaluated and reduced in
+  // turn.
   struct Batch {
     int64_t g0, g1, items;
   };
@@ -1087,21 +1198,8 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   std::vector<int64_t> ho, hc;
   CellEnt* clist;
   ChunkEnt* klist;
-  int32_t* ioff = s.get<int32_t>(ng + 1);
-  if (!ioff) return BH_ENOMEM;
   int64_t cell_int = 0, pair_int = 0, ntc = 0, ntk = 0, max_items = 0;
-  if (plan) {
-    // the one-pass lists are in place; their layout, the work items and the
-    // capacity checks come from the device
-    max_items = std::max<int64_t>(plan->cap_items, 1);
-    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, cpt, offs, ioff, dcap, max_items,
-                       err, plan->need);
-    BH_LAUNCH_CHECK();
-    clist = dclist;
-    klist = dklist;
-    batches.push_back({0, ng, max_items});
-    direct = true;
-  } else {
+  {
     hc.resize(3 * ng);
     for (;;) {
       uint32_t herr = 0;
@@ -1156,7 +1254,12 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     batches.back().items = bi;
     maxc = std::max(maxc, bc);
     maxk = std::max(maxk, bk);
-    for (const Batch& b : batches) max_items = std::max(max_items, b.items);
+    int64_t all_items = 0;
+    for (const Batch& b : batches) {
+      max_items = std::max(max_items, b.items);
+      all_items += b.items;
+    }
+    c->items_hint = std::max<int64_t>(1024, all_items + all_items / 4);
     if (!tgt) pair_int -= m;  // each target's own pair contributes nothing
     for (int64_t g = 0; g < ng; g++) {
       ntc += hc[g];
@@ -1171,17 +1274,12 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     }
   }
   if (!clist || !klist) return BH_ENOMEM;
-  const int tpg = gs;  // targets per group = 64 * TPL
   double* part = s.get<double>((size_t)std::max<int64_t>(max_items, 1) * tpg * 3);
-  uint32_t* next = s.get<uint32_t>(1);
-  if (!part || !next) return BH_ENOMEM;
-  const float eps2 = (float)(eps * eps);
-  const bool zeps = !(eps2 > 0.f);
-  const bool self = tgt == nullptr;
+  if (!part) return BH_ENOMEM;
   std::vector<int64_t> rel;
   std::vector<int32_t> reli, hord, bucket;
   int32_t* ord = nullptr;
-  if (!plan) {
+  {
     int64_t maxnb = 0;
     for (const Batch& b : batches) maxnb = std::max(maxnb, b.g1 - b.g0);
     ord = s.get<int32_t>((size_t)std::max<int64_t>(maxnb, 1));
@@ -1189,7 +1287,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   }
   for (const Batch& b : batches) {
     const int64_t nb = b.g1 - b.g0;
-    if (!plan && !direct) {
+    if (!direct) {
       // fill order: groups with the longest lists (the longest walks) first,
       // so the pass does not end on a few long walks started late; a
       // counting sort on the list length, in 4096 buckets
@@ -1208,7 +1306,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       for (int64_t g = b.g0; g < b.g1; g++) hord[bucket[4095 - ((hc[g] + hc[ng + g]) >> sh)]++] = (int32_t)(g - b.g0);
       BH_HIP(hipMemcpyAsync(ord, hord.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
     }
-    if (!plan) {
+    {
       // offsets of this batch's groups and items, relative to its first
       // list entries and first item
       rel.resize(2 * (nb + 1));
@@ -1233,31 +1331,9 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
                         m, gt0, ord);
       BH_LAUNCH_CHECK();
     }
-    {
-      int rc = fill_u32(c, next, 0u, 1);
-      if (rc) return rc;
-      ProfScope p(c, "k_group_eval_f32");
-      const int64_t* coff = offs;
-      const int64_t* koff = offs + ng + 1;
-      if (gs == 64) {
-        if (ch == 16) launch_eval<1, 16>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
-        else if (ch == 32) launch_eval<1, 32>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
-        else launch_eval<1, 64>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
-      } else {
-        if (ch == 16) launch_eval<2, 16>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
-        else if (ch == 32) launch_eval<2, 32>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
-        else launch_eval<2, 64>(c, b.g0, nb, ioff, zeps, self, kahan, quad, m, gs, tpos, t, boxes, coff, clist, koff, klist, cnt, cnt + ng, eps2, gt0, err, next, part, b.items);
-      }
-      BH_LAUNCH_CHECK();
-    }
-    {
-      ProfScope p(c, "k_item_reduce");
-      hipLaunchKernelGGL(k_item_reduce, dim3((unsigned)cdiv(nb * tpg, 256)), dim3(256), 0, c->stream, b.g0, nb, ioff,
-                         m, gs, gt0, part, tpg, torig, t->dim, G, err, out);
-      BH_LAUNCH_CHECK();
-    }
+    BH_TRY(evaluate(b.g0, nb, b.items, clist, klist, part));
   }
-  if (stats && !plan) {
+  if (stats) {
     stats[0] = cell_int;
     stats[1] = pair_int;
     stats[2] = ng;
