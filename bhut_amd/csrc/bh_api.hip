@@ -156,6 +156,16 @@ static int alloc_topology(bh_tree* t) {
   return BH_OK;
 }
 
+// The device system offsets of a single tree: {0, n} and {0, M}.
+__global__ void k_sys1(int32_t* poff, int32_t* noff, int32_t n, int32_t M) {
+  if (threadIdx.x == 0) {
+    poff[0] = 0;
+    poff[1] = n;
+    noff[0] = 0;
+    noff[1] = M;
+  }
+}
+
 // System offsets of a tree (nsys = 1) or forest, host and device copies.
 static int set_systems(bh_tree* t, const int64_t* counts, int64_t nsys) {
   bh_ctx* c = t->ctx;
@@ -170,6 +180,12 @@ static int set_systems(bh_tree* t, const int64_t* counts, int64_t nsys) {
   }
   if (!alloc_into(c, t->d_sys_poff, (size_t)nsys + 1) || !alloc_into(c, t->d_sys_noff, (size_t)nsys + 1))
     return BH_ENOMEM;
+  if (nsys == 1) {
+    // no host copy (and no synchronisation) for a single tree
+    hipLaunchKernelGGL(k_sys1, dim3(1), dim3(64), 0, c->stream, t->d_sys_poff, t->d_sys_noff, p32[1], n32[1]);
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+  }
   BH_HIP(hipMemcpyAsync(t->d_sys_poff, p32.data(), (nsys + 1) * 4, hipMemcpyHostToDevice, c->stream));
   BH_HIP(hipMemcpyAsync(t->d_sys_noff, n32.data(), (nsys + 1) * 4, hipMemcpyHostToDevice, c->stream));
   BH_HIP(hipStreamSynchronize(c->stream));  // the host vectors are temporaries

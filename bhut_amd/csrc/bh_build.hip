@@ -176,6 +176,15 @@ int forest_plan(const int64_t* counts, int64_t nsys, int64_t L, std::vector<int6
   return BH_OK;
 }
 
+// A single tree's tables, stored from the kernel arguments (no host copy to
+// wait for).
+__global__ void k_put_tables(TopoTables tb, SysInfo si, TopoTables* __restrict__ d_tb, SysInfo* __restrict__ d_si) {
+  if (threadIdx.x == 0) {
+    *d_tb = tb;
+    *d_si = si;
+  }
+}
+
 int topology(bh_ctx* c, bh_tree* t) {
   const int64_t L = t->leaf, nsys = t->nsys;
   std::vector<TopoTables> tbs(nsys);
@@ -197,14 +206,19 @@ int topology(bh_ctx* c, bh_tree* t) {
   TopoTables* d_tb = sc.get<TopoTables>(nsys);
   SysInfo* d_info = sc.get<SysInfo>(nsys);
   if (!d_tb || !d_info) return BH_ENOMEM;
-  BH_HIP(hipMemcpyAsync(d_tb, tbs.data(), nsys * sizeof(TopoTables), hipMemcpyHostToDevice, c->stream));
-  BH_HIP(hipMemcpyAsync(d_info, info.data(), nsys * sizeof(SysInfo), hipMemcpyHostToDevice, c->stream));
+  if (nsys == 1) {
+    hipLaunchKernelGGL(k_put_tables, dim3(1), dim3(64), 0, c->stream, tbs[0], info[0], d_tb, d_info);
+    BH_LAUNCH_CHECK();
+  } else {
+    BH_HIP(hipMemcpyAsync(d_tb, tbs.data(), nsys * sizeof(TopoTables), hipMemcpyHostToDevice, c->stream));
+    BH_HIP(hipMemcpyAsync(d_info, info.data(), nsys * sizeof(SysInfo), hipMemcpyHostToDevice, c->stream));
+  }
   const int nb = (int)std::min<int64_t>(cdiv(slots, 256), 65536);
   hipLaunchKernelGGL(k_topology, dim3(nb), dim3(256), 0, c->stream, nsys, L, slots, d_info, d_tb, t->start,
                      t->count, t->first_child, t->parent, t->skip, t->leaf_order);
   BH_LAUNCH_CHECK();
-  // the host tables must outlive the asynchronous copies
-  BH_HIP(hipStreamSynchronize(c->stream));
+  // the host tables of a forest must outlive the asynchronous copies
+  if (nsys > 1) BH_HIP(hipStreamSynchronize(c->stream));
   return BH_OK;
 }
 
