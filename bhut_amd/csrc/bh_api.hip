@@ -408,11 +408,9 @@ int bh_tree_build_dev(bh_ctx* c, const double* d_pos, const double* d_mass, int6
   DeviceGuard g(c->device);
   bh_tree* t = new bh_tree();
   const int64_t counts[1] = {n};
+  // enqueued on the context stream without waiting for it (like every _dev
+  // entry point): the work that uses the tree follows on the same stream
   int rc = build_dev(c, d_pos, d_mass, counts, 1, dim, leaf_size, t);
-  if (rc == BH_OK) {
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) rc = hip_fail(e, "build", __FILE__, __LINE__);
-  }
   if (rc != BH_OK) {
     free_tree_arrays(t);
     delete t;
@@ -439,7 +437,15 @@ int bh_tree_build(bh_ctx* c, const double* pos, const double* mass, int64_t n, i
   if (!dp || !dm) return BH_ENOMEM;
   BH_HIP(hipMemcpyAsync(dp, pos, (size_t)n * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
   BH_HIP(hipMemcpyAsync(dm, mass, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  return bh_tree_build_dev(c, dp, dm, n, dim, leaf_size, out);
+  BH_TRY(bh_tree_build_dev(c, dp, dm, n, dim, leaf_size, out));
+  // host entry points return with their work done
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    bh_tree_free(*out);
+    *out = nullptr;
+    return hip_fail(e, "build", __FILE__, __LINE__);
+  }
+  return BH_OK;
 }
 
 int bh_forest_build_dev(bh_ctx* c, const double* d_pos, const double* d_mass, const int64_t* counts,
@@ -462,10 +468,6 @@ int bh_forest_build_dev(bh_ctx* c, const double* d_pos, const double* d_mass, co
   DeviceGuard g(c->device);
   bh_tree* t = new bh_tree();
   int rc = build_dev(c, d_pos, d_mass, counts, nsys, dim, leaf_size, t);
-  if (rc == BH_OK) {
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) rc = hip_fail(e, "forest build", __FILE__, __LINE__);
-  }
   if (rc != BH_OK) {
     free_tree_arrays(t);
     delete t;
@@ -489,7 +491,14 @@ int bh_forest_build(bh_ctx* c, const double* pos, const double* mass, const int6
   if (!dp || !dm) return BH_ENOMEM;
   BH_HIP(hipMemcpyAsync(dp, pos, (size_t)n * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
   BH_HIP(hipMemcpyAsync(dm, mass, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  return bh_forest_build_dev(c, dp, dm, counts, nsys, dim, leaf_size, out);
+  BH_TRY(bh_forest_build_dev(c, dp, dm, counts, nsys, dim, leaf_size, out));
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    bh_tree_free(*out);
+    *out = nullptr;
+    return hip_fail(e, "forest build", __FILE__, __LINE__);
+  }
+  return BH_OK;
 }
 
 int bh_tree_systems(const bh_tree* t, int64_t* nsys, int64_t* particle_offsets, int64_t* node_offsets) {

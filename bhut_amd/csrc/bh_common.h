@@ -101,6 +101,10 @@ struct bh_ctx {
   int64_t list_hint = 8192;
   // work items of the previous host-scanned FP32 call, with a margin
   int64_t items_hint = 65536;
+  // group order for the next host-scanned FP32 walk (longest lists of the
+  // previous call first); valid for walk_order_n groups
+  int32_t* walk_order = nullptr;
+  int64_t walk_order_cap = 0, walk_order_n = 0;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   struct ProfRec {
@@ -247,8 +251,8 @@ int f32_prepare(bh_ctx* c, bh_tree* t);
 // items; an overflow sets bit 2 of *err and need[] = (longest list, items).
 struct F32Plan {
   int64_t cap_region = 0, cap_items = 0;
-  uint32_t* err = nullptr;  // device
-  int64_t* need = nullptr;  // device, 2 entries
+  uint32_t* err = nullptr;    // device
+  int64_t* need = nullptr;    // device, 2 entries
 };
 int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, double eps,
              double G, const bh_f32_opts* o, double* out, int64_t* stats, const F32Plan* plan = nullptr);

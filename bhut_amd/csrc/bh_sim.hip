@@ -217,6 +217,9 @@ int bh_sim_create(bh_ctx* c, double* d_pos, double* d_vel, const double* d_mass,
   if (!s->opts.stack_cap) s->opts.stack_cap = 1024;
   if (!s->opts.multipole) s->opts.multipole = 2;
   int rc = bh_tree_build_dev(c, d_pos, d_mass, n, dim, leaf_size, &s->tree);
+  // the simulation's own stream takes over from here
+  if (rc == BH_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = hip_fail(hipGetLastError(), "build", __FILE__, __LINE__);
   if (rc == BH_OK && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     rc = hip_fail(hipGetLastError(), "hipStreamCreateWithFlags", __FILE__, __LINE__);
   if (rc == BH_OK) {

@@ -75,7 +75,7 @@ int bh_ctx_profile_summary(bh_ctx *ctx, char *buf, int64_t cap);
 int bh_tree_build(bh_ctx *ctx, const double *pos, const double *mass, int64_t n, int32_t dim,
                   int32_t leaf_size, bh_tree **out);
 int bh_tree_build_dev(bh_ctx *ctx, const double *d_pos, const double *d_mass, int64_t n,
-                      int32_t dim, int32_t leaf_size, bh_tree **out); /* synchronises */
+                      int32_t dim, int32_t leaf_size, bh_tree **out); /* enqueued, returns at once */
 /* TreeData (bhut/tree/node.py:15-77) in host memory plus the source
  * positions/masses the traversal will use (bh.py:428 `positions, masses`). */
 int bh_tree_import(bh_ctx *ctx, int64_t n, int64_t num_nodes, int32_t dim, int32_t leaf_size,
@@ -94,7 +94,7 @@ int bh_tree_clone(const bh_tree *t, bh_tree **out);
 int bh_forest_build(bh_ctx *ctx, const double *pos, const double *mass, const int64_t *counts, int64_t nsys,
                     int32_t dim, int32_t leaf_size, bh_tree **out);
 int bh_forest_build_dev(bh_ctx *ctx, const double *d_pos, const double *d_mass, const int64_t *counts,
-                        int64_t nsys, int32_t dim, int32_t leaf_size, bh_tree **out); /* synchronises */
+                        int64_t nsys, int32_t dim, int32_t leaf_size, bh_tree **out); /* enqueued */
 /* Number of systems and their particle / node offsets (nsys + 1 entries
  * each; either array may be NULL). */
 int bh_tree_systems(const bh_tree *t, int64_t *nsys, int64_t *particle_offsets, int64_t *node_offsets);
