@@ -105,6 +105,10 @@ struct bh_ctx {
   // previous call first); valid for walk_order_n groups
   int32_t* walk_order = nullptr;
   int64_t walk_order_cap = 0, walk_order_n = 0;
+  // list lengths of the previous host-scanned FP32 call (cells, then leaves;
+  // prev_ng groups) and their totals: region sizes for the next one-pass walk
+  int64_t* prev_cnt = nullptr;
+  int64_t prev_cap = 0, prev_ng = -1, prev_ntc = 0, prev_ntk = 0;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   struct ProfRec {

@@ -380,7 +380,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
         if (o < ke) klist[o] = v;
       }
     }
-    if ((!FILL || cap) && kind == 2) {
+    if (ccount && kind == 2) {
       np += nd.count;  // this lane's share; summed over the lanes at the end
     }
     nc += __popcll(bc);
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_group_walk(const NodeD* __r
     sp += push;
     wave_sync();
   }
-  if (!FILL || cap) {
+  if (ccount) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) np += __shfl_xor(np, off);
     if (lane == 0) {
@@ -791,16 +791,18 @@ __global__ __launch_bounds__(256) void k_item_reduce(int64_t g0, int64_t ngroups
   if (dim == 3) out[o * dim + 2] = G * a2;
 }
 
-// The planned path's list layout and work items, on the device: each
-// group's lists sit in regions of cap entries (coff[g] = koff[g] = g * cap),
-// ioff[] is the prefix of the groups' item counts.  A list longer than its
-// region, or more items than cap_items, sets bit 2 of *err (the evaluation
-// then does nothing); need[0] = the longest list, need[1] = the items.
+// The one-pass lists' layout and work items, on the device: each group's
+// lists sit in regions of cap entries (coff[g] = koff[g] = g * cap), or
+// (preset) in the regions offs already holds; ioff[] is the prefix of the
+// groups' item counts.  A list longer than its region, or more items than
+// cap_items, sets bit 2 of *err (the evaluation then does nothing);
+// need[0] = the longest list, need[1] = the items.
 // sums (optional): cell and pair interactions (entries x the group's
 // targets), cell and leaf entries.
 __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* __restrict__ cnt, int cpt,
                                                      int64_t* __restrict__ offs, int32_t* __restrict__ ioff,
-                                                     int64_t cap, int64_t cap_items, uint32_t* __restrict__ err,
+                                                     int64_t cap, int preset, int64_t cap_items,
+                                                     uint32_t* __restrict__ err,
                                                      int64_t* __restrict__ need, int64_t* __restrict__ sums,
                                                      int64_t m, int gs, const int32_t* __restrict__ gt0) {
   constexpr int IT = 8;
@@ -810,6 +812,7 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
   int64_t* koff = offs + ng + 1;
   int64_t carry_i = 0, mx = 0;
   int64_t acc[4] = {0, 0, 0, 0};
+  bool ovf = false;
   for (int64_t base = 0; base < ng; base += 1024 * IT) {
     int64_t vi[IT];
     int64_t si = 0;
@@ -829,6 +832,8 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
       }
       mx = vc > mx ? vc : mx;
       mx = vk > mx ? vk : mx;
+      // preset regions: a list longer than its own region overflows
+      if (preset && g < ng && (vc > coff[g + 1] - coff[g] || vk > koff[g + 1] - koff[g])) ovf = true;
     }
     int64_t ii = si;  // inclusive wave scan of the thread sums
 #pragma unroll
@@ -848,8 +853,10 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
     for (int i = 0; i < IT; i++) {
       const int64_t g = base + (int64_t)tid * IT + i;
       if (g < ng) {
-        coff[g] = g * cap;
-        koff[g] = g * cap;
+        if (!preset) {
+          coff[g] = g * cap;
+          koff[g] = g * cap;
+        }
         ioff[g] = (int32_t)pi;
       }
       pi += vi[i];
@@ -878,12 +885,69 @@ __global__ __launch_bounds__(1024) void k_scan_lists(int64_t ng, const int64_t* 
         for (int j = 0; j < 16; j++) t += wsums[q][j];
         sums[q] = t;
       }
-    coff[ng] = ng * cap;
-    koff[ng] = ng * cap;
+    if (!preset) {
+      coff[ng] = ng * cap;
+      koff[ng] = ng * cap;
+    }
     ioff[ng] = (int32_t)carry_i;
     need[0] = mx;
     need[1] = carry_i;
-    if (mx > cap || carry_i > cap_items) atomicOr(err, 2u);
+    if ((!preset && mx > cap) || carry_i > cap_items) atomicOr(err, 2u);
+  }
+  if (ovf) atomicOr(err, 2u);
+}
+
+// Regions for a one-pass walk from the previous call's list lengths (same
+// groups): prev + prev/4 + 64 entries per group and list, as offsets in offs
+// (cells, then leaves; ng + 1 each).  One workgroup.
+__global__ __launch_bounds__(1024) void k_prev_regions(int64_t ng, const int64_t* __restrict__ prev,
+                                                       int64_t* __restrict__ offs) {
+  constexpr int IT = 8;
+  __shared__ int64_t wsum[2][16];
+  const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+  int64_t carry[2] = {0, 0};
+  for (int64_t base = 0; base < ng; base += 1024 * IT) {
+    int64_t v[2][IT], sum[2] = {0, 0};
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+      const int64_t g = base + (int64_t)tid * IT + i;
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        const int64_t p = g < ng ? prev[l * ng + g] : -1;
+        v[l][i] = p >= 0 ? p + p / 4 + 64 : 0;
+        sum[l] += v[l][i];
+      }
+    }
+    int64_t inc[2] = {sum[0], sum[1]};
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        const int64_t y = __shfl_up(inc[l], off);
+        if (lane >= off) inc[l] += y;
+      }
+    if (lane == 63)
+      for (int l = 0; l < 2; l++) wsum[l][w] = inc[l];
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+      int64_t pre = carry[l] + inc[l] - sum[l], tot = 0;
+      for (int j = 0; j < 16; j++) {
+        if (j < w) pre += wsum[l][j];
+        tot += wsum[l][j];
+      }
+      for (int i = 0; i < IT; i++) {
+        const int64_t g = base + (int64_t)tid * IT + i;
+        if (g < ng) offs[l * (ng + 1) + g] = pre;
+        pre += v[l][i];
+      }
+      carry[l] += tot;
+    }
+    __syncthreads();  // wsum is rewritten by the next chunk
+  }
+  if (tid == 0) {
+    offs[ng] = carry[0];
+    offs[2 * ng + 1] = carry[1];
   }
 }
 
@@ -942,6 +1006,14 @@ __global__ __launch_bounds__(256) void k_order_scatter(int64_t ng, const int64_t
 // unless the context sets another budget: a quarter of the device memory, at
 // most 64 GiB (12 GiB when the device memory is unknown).
 constexpr int64_t LIST_HINT_MAX = 65536;  // one-pass list region, entries per group at most
+
+// Region size for the next host-scanned call from the longest list of this
+// one: a quarter more, in multiples of 1024 entries; 0 (count, then fill
+// packed lists) when that exceeds LIST_HINT_MAX.
+static int64_t next_list_hint(int64_t longest) {
+  const int64_t r = std::max<int64_t>(1024, (longest + longest / 4 + 1023) & ~int64_t(1023));
+  return r <= LIST_HINT_MAX ? r : 0;
+}
 
 static size_t default_list_budget(const bh_ctx* c) {
   if (c->mem_total == 0) return size_t(12) << 30;
@@ -1106,7 +1178,8 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   // if those regions fit the list budget (dcap: the context's longest list of
   // its previous call, with a margin).  Otherwise -- or when a list outgrows
   // its region -- the groups are counted first and filled at scanned offsets.
-  int64_t dcap = 0;
+  int64_t dcap = 0;     // uniform one-pass regions of dcap entries per group
+  bool sized = false;   // one-pass regions sized per group (offsets in offs)
   CellEnt* dclist = nullptr;
   ChunkEnt* dklist = nullptr;
   // the host-scanned one-pass walk visits the groups in the order the
@@ -1135,9 +1208,25 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     dklist = s.get<ChunkEnt>((size_t)(ng * dcap));
     if (!dclist || !dklist) return BH_ENOMEM;
   } else {
+    // regions from the previous call's list lengths when it had the same
+    // groups (a quarter more each: memory ~1.25x the lists), else uniform
+    // regions from its longest list
     const size_t budget = c->list_budget ? c->list_budget : default_list_budget(c);
+    if (c->prev_cnt && c->prev_ng == ng) {
+      const int64_t tc = c->prev_ntc + c->prev_ntc / 4 + 64 * ng, tk = c->prev_ntk + c->prev_ntk / 4 + 64 * ng;
+      if ((size_t)(tc + tk) * sizeof(int32_t) <= budget) {
+        dclist = s.get<CellEnt>((size_t)tc);
+        dklist = s.get<ChunkEnt>((size_t)tk);
+        if (dclist && dklist) {
+          hipLaunchKernelGGL(k_prev_regions, dim3(1), dim3(1024), 0, c->stream, ng, c->prev_cnt, offs);
+          BH_LAUNCH_CHECK();
+          sized = true;
+        }
+        (void)hipGetLastError();
+      }
+    }
     const int64_t want = c->list_hint;
-    if ((size_t)ng * (size_t)want * 2 * sizeof(int32_t) <= budget) {
+    if (!sized && want > 0 && (size_t)ng * (size_t)want * 2 * sizeof(int32_t) <= budget) {
       dclist = s.get<CellEnt>((size_t)(ng * want));
       dklist = s.get<ChunkEnt>((size_t)(ng * want));
       if (dclist && dklist) dcap = want;
@@ -1150,10 +1239,10 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     const int64_t per_wave = (int64_t)stack_cap * 4 + CBUF * (int64_t)sizeof(double4);
     wpb = (int)std::max<int64_t>(1, std::min<int64_t>(WALK_WAVES, max_lds / per_wave));
     lds = (size_t)wpb * per_wave;
-    if (dcap) {
+    if (dcap || sized) {
       ProfScope p(c, "k_group_walk_fill");
       launch_walk<true>(c, t, dim3((unsigned)cdiv(ng, wpb)), lds, gs, boxes, itheta2, stack_cap, cnt, cnt + ng,
-                        cnt + 2 * ng, nullptr, nullptr, dclist, dklist, err, groot, 0, ng, wpb, tpos, m, gt0,
+                        cnt + 2 * ng, offs, offs + ng + 1, dclist, dklist, err, groot, 0, ng, wpb, tpos, m, gt0,
                         word, dcap);
     } else {
       ProfScope p(c, "k_group_walk_count");
@@ -1209,7 +1298,23 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     }
     return BH_OK;
   };
-  if (plan || dcap > 0) {
+  // the list lengths of this call, kept for the next one's regions
+  auto keep_counts = [&](int64_t ntc_, int64_t ntk_) -> int {
+    if (c->prev_cap < 2 * ng) {
+      dfree(c, c->prev_cnt);
+      c->prev_cnt = dalloc_t<int64_t>(c, (size_t)(2 * ng));
+      c->prev_cap = c->prev_cnt ? 2 * ng : 0;
+      c->prev_ng = -1;
+      (void)hipGetLastError();
+    }
+    if (!c->prev_cnt) return BH_OK;
+    BH_HIP(hipMemcpyAsync(c->prev_cnt, cnt, (size_t)(2 * ng) * sizeof(int64_t), hipMemcpyDeviceToDevice, c->stream));
+    c->prev_ng = ng;
+    c->prev_ntc = ntc_;
+    c->prev_ntk = ntk_;
+    return BH_OK;
+  };
+  if (plan || dcap > 0 || sized) {
     // The one-pass lists are in place: their layout, the work items and the
     // capacity checks come from the device, and the evaluation follows at
     // once (it does nothing if a check failed).  A planned walk stops here;
@@ -1219,7 +1324,8 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     int64_t* need = plan ? plan->need : s.get<int64_t>(6);
     double* part = s.get<double>((size_t)cap_items * tpg * 3);
     if (!need || !part) return BH_ENOMEM;
-    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, cpt, offs, ioff, dcap, cap_items,
+    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, c->stream, ng, cnt, cpt, offs, ioff, dcap, (int)sized,
+                       cap_items,
                        err, need, plan ? nullptr : need + 2, m, gs, gt0);
     BH_LAUNCH_CHECK();
     if (wnext) {
@@ -1243,7 +1349,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     if (!(herr & 1)) {
       // regions and item buffer for the next call: what this one needed and
       // a quarter more
-      c->list_hint = std::min<int64_t>(LIST_HINT_MAX, std::max<int64_t>(1024, (h[0] + h[0] / 4 + 1023) & ~int64_t(1023)));
+      c->list_hint = next_list_hint(h[0]);
       c->items_hint = std::max<int64_t>(1024, h[1] + h[1] / 4);
     }
     if (!herr) {
@@ -1253,7 +1359,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
         stats[2] = ng;
         stats[3] = h[4] + h[5];
       }
-      return BH_OK;
+      return keep_counts(h[4], h[5]);
     }
     BH_TRY(fill_u32(c, err, 0u, 1));
     if (herr & 1) {
@@ -1302,7 +1408,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     direct = dcap > 0 && longest <= dcap;
     // regions for the next call: the longest list and a quarter more, in
     // multiples of 1024 entries
-    c->list_hint = std::min<int64_t>(LIST_HINT_MAX, std::max<int64_t>(1024, (longest + longest / 4 + 1023) & ~int64_t(1023)));
+    c->list_hint = next_list_hint(longest);
     ho.assign(2 * (ng + 1), 0);
     int64_t maxc = 0, maxk = 0, bc = 0, bk = 0, bi = 0;
     size_t bytes = 0;
@@ -1418,7 +1524,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     stats[2] = ng;
     stats[3] = ntc + ntk;
   }
-  return BH_OK;
+  return keep_counts(ntc, ntk);
 }
 
 }  // namespace bh

@@ -123,12 +123,14 @@ def test_api_precision_keyword(native):
 
 
 def test_one_pass_lists_match_counted_lists(native):
-    """The lists are written in one pass into per-group regions sized by the
-    context's previous call, or -- when a list outgrows its region, or the
-    regions exceed the list budget -- counted first and packed.  At theta = 0
-    every leaf is in every group's list (16384 leaves > the first call's 8192-
-    entry regions), so a fresh context takes the counted path first and the
-    one-pass path next; the lists and so the results are bit-identical."""
+    """The lists are written in one pass into regions sized from the
+    context's previous call (per group when it had the same groups, else
+    uniform from its longest list), or -- when a list outgrows its region, or
+    the regions exceed the list budget -- counted first and packed.  At
+    theta = 0 every leaf is in every group's list (16384 leaves > a fresh
+    context's 8192-entry uniform regions), so the first call takes the counted
+    path and the second the one-pass path; the lists and so the results are
+    bit-identical."""
     x, m = workloads.plummer(300_000, seed=13)
     c = native.Context(0)
     t = native.DeviceTree.build(x, m, 32, context=c)
