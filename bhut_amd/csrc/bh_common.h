@@ -96,9 +96,6 @@ struct bh_ctx {
   bool capturing = false;
   // bytes of FP32 interaction lists materialised at once (0: default)
   size_t list_budget = 0;
-  // entries per group of the FP32 one-pass list regions: the longest list of
-  // the previous host-scanned call, with a margin
-  int64_t list_hint = 8192;
   // work items of the previous host-scanned FP32 call, with a margin
   int64_t items_hint = 65536;
   // group order for the next host-scanned FP32 walk (longest lists of the

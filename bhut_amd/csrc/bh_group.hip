@@ -1005,15 +1005,6 @@ __global__ __launch_bounds__(256) void k_order_scatter(int64_t ng, const int64_t
 // Bytes of interaction lists materialised at once by the host-scanned path
 // unless the context sets another budget: a quarter of the device memory, at
 // most 64 GiB (12 GiB when the device memory is unknown).
-constexpr int64_t LIST_HINT_MAX = 65536;  // one-pass list region, entries per group at most
-
-// Region size for the next host-scanned call from the longest list of this
-// one: a quarter more, in multiples of 1024 entries; 0 (count, then fill
-// packed lists) when that exceeds LIST_HINT_MAX.
-static int64_t next_list_hint(int64_t longest) {
-  const int64_t r = std::max<int64_t>(1024, (longest + longest / 4 + 1023) & ~int64_t(1023));
-  return r <= LIST_HINT_MAX ? r : 0;
-}
 
 static size_t default_list_budget(const bh_ctx* c) {
   if (c->mem_total == 0) return size_t(12) << 30;
@@ -1173,11 +1164,13 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   }
   int wpb = 1;
   size_t lds = 0;
-  // One pass when it fits: the host-scanned path writes the lists straight
-  // into regions of dcap entries per group, counting them at the same time,
-  // if those regions fit the list budget (dcap: the context's longest list of
-  // its previous call, with a margin).  Otherwise -- or when a list outgrows
-  // its region -- the groups are counted first and filled at scanned offsets.
+  // One pass when it can: the walk writes the lists straight into regions of
+  // their own and counts them at the same time -- a planned walk into
+  // uniform regions of dcap entries per group, a host-scanned one into
+  // regions sized per group from the context's previous call on the same
+  // groups, if they fit the list budget.  Otherwise (no such call, or a list
+  // outgrew its region) the groups are counted first and filled at scanned
+  // offsets.
   int64_t dcap = 0;     // uniform one-pass regions of dcap entries per group
   bool sized = false;   // one-pass regions sized per group (offsets in offs)
   CellEnt* dclist = nullptr;
@@ -1209,8 +1202,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     if (!dclist || !dklist) return BH_ENOMEM;
   } else {
     // regions from the previous call's list lengths when it had the same
-    // groups (a quarter more each: memory ~1.25x the lists), else uniform
-    // regions from its longest list
+    // groups (a quarter more each: memory ~1.25x the lists)
     const size_t budget = c->list_budget ? c->list_budget : default_list_budget(c);
     if (c->prev_cnt && c->prev_ng == ng) {
       const int64_t tc = c->prev_ntc + c->prev_ntc / 4 + 64 * ng, tk = c->prev_ntk + c->prev_ntk / 4 + 64 * ng;
@@ -1224,13 +1216,6 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
         }
         (void)hipGetLastError();
       }
-    }
-    const int64_t want = c->list_hint;
-    if (!sized && want > 0 && (size_t)ng * (size_t)want * 2 * sizeof(int32_t) <= budget) {
-      dclist = s.get<CellEnt>((size_t)(ng * want));
-      dklist = s.get<ChunkEnt>((size_t)(ng * want));
-      if (dclist && dklist) dcap = want;
-      (void)hipGetLastError();
     }
   }
   // the count (or one-pass) walk; an LDS stack overflow on the host-scanned
@@ -1253,7 +1238,6 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     BH_LAUNCH_CHECK();
     return BH_OK;
   };
-  bool direct = false;  // the lists are in place after the one pass
   {
     int rc = count_pass();
     if (rc) return rc;
@@ -1346,12 +1330,9 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     BH_HIP(hipMemcpyAsync(h, need, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     BH_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
     BH_HIP(hipStreamSynchronize(c->stream));
-    if (!(herr & 1)) {
-      // regions and item buffer for the next call: what this one needed and
-      // a quarter more
-      c->list_hint = next_list_hint(h[0]);
-      c->items_hint = std::max<int64_t>(1024, h[1] + h[1] / 4);
-    }
+    // the item buffer for the next call: what this one needed and a quarter
+    // more
+    if (!(herr & 1)) c->items_hint = std::max<int64_t>(1024, h[1] + h[1] / 4);
     if (!herr) {
       if (stats) {
         stats[0] = h[2];
@@ -1403,25 +1384,18 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       if (rc) return rc;
     }
     const size_t budget = c->list_budget ? c->list_budget : default_list_budget(c);
-    int64_t longest = 0;
-    for (int64_t g = 0; g < ng; g++) longest = std::max(longest, std::max(hc[g], hc[ng + g]));
-    direct = dcap > 0 && longest <= dcap;
-    // regions for the next call: the longest list and a quarter more, in
-    // multiples of 1024 entries
-    c->list_hint = next_list_hint(longest);
     ho.assign(2 * (ng + 1), 0);
     int64_t maxc = 0, maxk = 0, bc = 0, bk = 0, bi = 0;
     size_t bytes = 0;
     batches.push_back({0, 0, 0});
     for (int64_t g = 0; g < ng; g++) {
-      // one-pass lists start at g * dcap; scanned lists are packed
-      ho[g + 1] = direct ? (g + 1) * dcap : ho[g] + hc[g];
-      ho[ng + 1 + g + 1] = direct ? (g + 1) * dcap : ho[ng + 1 + g] + hc[ng + g];
+      ho[g + 1] = ho[g] + hc[g];
+      ho[ng + 1 + g + 1] = ho[ng + 1 + g] + hc[ng + g];
       const int64_t tg = gt0 ? t->h_gtab[g + 1] - t->h_gtab[g] : std::min<int64_t>(gs, m - g * gs);
       cell_int += hc[g] * tg;
       pair_int += hc[2 * ng + g] * tg;
       const size_t gb = hc[g] * sizeof(CellEnt) + hc[ng + g] * sizeof(ChunkEnt);
-      if (!direct && g > batches.back().g0 && bytes + gb > budget) {
+      if (g > batches.back().g0 && bytes + gb > budget) {
         batches.back().g1 = g;
         batches.back().items = bi;
         batches.push_back({g, 0, 0});
@@ -1450,13 +1424,8 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       ntc += hc[g];
       ntk += hc[ng + g];
     }
-    if (direct || (dcap && maxc <= ng * dcap && maxk <= ng * dcap)) {
-      clist = dclist;  // the one-pass regions (in place, or reused for the packed lists)
-      klist = dklist;
-    } else {
-      clist = s.get<CellEnt>((size_t)std::max<int64_t>(maxc, 1));
-      klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(maxk, 1));
-    }
+    clist = s.get<CellEnt>((size_t)std::max<int64_t>(maxc, 1));
+    klist = s.get<ChunkEnt>((size_t)std::max<int64_t>(maxk, 1));
   }
   if (!clist || !klist) return BH_ENOMEM;
   double* part = s.get<double>((size_t)std::max<int64_t>(max_items, 1) * tpg * 3);
@@ -1472,7 +1441,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   }
   for (const Batch& b : batches) {
     const int64_t nb = b.g1 - b.g0;
-    if (!direct) {
+    {
       // fill order: groups with the longest lists (the longest walks) first,
       // so the pass does not end on a few long walks started late; a
       // counting sort on the list length, in 4096 buckets
@@ -1509,7 +1478,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       BH_HIP(hipMemcpyAsync(ioff, reli.data(), (nb + 1) * 4, hipMemcpyHostToDevice, c->stream));
       BH_HIP(hipStreamSynchronize(c->stream));  // rel, reli, hord are reused by the next batch
     }
-    if (!direct) {
+    {
       ProfScope p(c, "k_group_walk_fill");
       launch_walk<true>(c, t, dim3((unsigned)cdiv(nb, wpb)), lds, gs, boxes, itheta2, stack_cap,
                         nullptr, nullptr, nullptr, offs, offs + ng + 1, clist, klist, err, groot, b.g0, nb, wpb, tpos,

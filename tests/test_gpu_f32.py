@@ -123,23 +123,26 @@ def test_api_precision_keyword(native):
 
 
 def test_one_pass_lists_match_counted_lists(native):
-    """The lists are written in one pass into regions sized from the
-    context's previous call (per group when it had the same groups, else
-    uniform from its longest list), or -- when a list outgrows its region, or
-    the regions exceed the list budget -- counted first and packed.  At
-    theta = 0 every leaf is in every group's list (16384 leaves > a fresh
-    context's 8192-entry uniform regions), so the first call takes the counted
-    path and the second the one-pass path; the lists and so the results are
-    bit-identical."""
+    """A context's first call on a set of groups counts the lists, then fills
+    them packed; later calls on the same groups write them in one pass into
+    regions sized from the previous call's list lengths -- or count and pack
+    again when the regions exceed the list budget.  At theta = 0 every leaf
+    is in every group's list.  The lists, and so the results, are
+    bit-identical on every path, also when regions sized from shorter lists
+    overflow."""
     x, m = workloads.plummer(300_000, seed=13)
     c = native.Context(0)
     t = native.DeviceTree.build(x, m, 32, context=c)
-    first = t.accel_f32(None, 0.0, 0.01, 1.0)   # regions too small: counted and packed
+    first = t.accel_f32(None, 0.0, 0.01, 1.0)   # no previous call: counted and packed
     second = t.accel_f32(None, 0.0, 0.01, 1.0)  # regions sized by the first call: one pass
     np.testing.assert_array_equal(second.view(np.uint64), first.view(np.uint64))
     c.set_list_budget(1 << 20)                   # regions over budget: counted, in batches
     third = t.accel_f32(None, 0.0, 0.01, 1.0)
     np.testing.assert_array_equal(third.view(np.uint64), first.view(np.uint64))
+    c.set_list_budget(0)
+    t.accel_f32(None, 0.5, 0.01, 1.0)            # short lists
+    fourth = t.accel_f32(None, 0.0, 0.01, 1.0)  # regions from the short lists overflow: counted again
+    np.testing.assert_array_equal(fourth.view(np.uint64), first.view(np.uint64))
     d = native.direct(x[:2000], m[:2000], softening=0.01)
     sub = native.DeviceTree.build(x[:2000], m[:2000], 32, context=c).accel_f32(None, 0.0, 0.01, 1.0)
     assert rel_err(sub, d)[0] < 2e-6
