@@ -132,7 +132,8 @@ static void free_tree_arrays(bh_tree* t) {
   bh_ctx* c = t->ctx;
   void* ps[] = {t->spart, t->perm,  t->start, t->count, t->first_child, t->parent, t->skip,
                 t->leaf_order, t->center, t->half, t->mass, t->com, t->bmin, t->bmax, t->gbox,
-                t->nodes, t->spart32, t->pleaf, t->quad, t->quad32, t->d_sys_poff, t->d_sys_noff, t->d_gtab};
+                t->nodes, t->spart32, t->pleaf, t->quad, t->quad32, t->d_sys_poff, t->d_sys_noff, t->d_gtab,
+                t->slot_node, t->d_lvl_sys, t->d_slot_off};
   for (void* p : ps) dfree(c, p);
 }
 

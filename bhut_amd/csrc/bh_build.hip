@@ -45,7 +45,8 @@ __global__ __launch_bounds__(256) void k_topology(int64_t nsys, int64_t L, int64
                                                   int32_t* __restrict__ first_child,
                                                   int32_t* __restrict__ parent,
                                                   int32_t* __restrict__ skip,
-                                                  int32_t* __restrict__ leaf_order) {
+                                                  int32_t* __restrict__ leaf_order,
+                                                  int32_t* __restrict__ slot_node) {
   for (int64_t gslot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gslot < nslots;
        gslot += (int64_t)gridDim.x * blockDim.x) {
     int64_t lo = 0, hi = nsys - 1;  // last system whose slots start at or before gslot
@@ -87,8 +88,12 @@ __global__ __launch_bounds__(256) void k_topology(int64_t nsys, int64_t L, int64
       par = idx;
       idx = nidx;
     }
-    if (!ok) continue;
+    if (!ok) {
+      slot_node[gslot] = -1;  // below a leaf
+      continue;
+    }
     const int64_t gi = idx + S.noff;
+    slot_node[gslot] = (int32_t)gi;
     start[gi] = (int32_t)(s + S.poff);
     count[gi] = (int32_t)c;
     parent[gi] = par >= 0 ? (int32_t)(par + S.noff) : -1;
@@ -189,18 +194,53 @@ int topology(bh_ctx* c, bh_tree* t) {
   const int64_t L = t->leaf, nsys = t->nsys;
   std::vector<TopoTables> tbs(nsys);
   std::vector<SysInfo> info(nsys);
+  std::vector<int> depth(nsys);
   int64_t slots = 0;
   for (int64_t s = 0; s < nsys; s++) {
     const int64_t N = t->sys_poff[s + 1] - t->sys_poff[s];
     int D = 0;
     int rc = topo_tables(N, L, tbs[s], D);
     if (rc) return rc;
+    depth[s] = D;
     info[s].N = N;
     info[s].slot_off = slots;
     info[s].noff = t->sys_noff[s];
     info[s].poff = t->sys_poff[s];
     info[s].loff = t->sys_loff[s];
     slots += (2ll << D) - 1;
+  }
+  // depth tables of the level-by-level upward passes: per depth, the systems
+  // deep enough (and, for a forest, each system's first slot)
+  dfree(c, t->slot_node);
+  dfree(c, t->d_lvl_sys);
+  dfree(c, t->d_slot_off);
+  t->slot_node = nullptr;
+  t->d_lvl_sys = nullptr;
+  t->d_slot_off = nullptr;
+  t->dmax = 0;
+  for (int64_t s = 0; s < nsys; s++) t->dmax = std::max(t->dmax, depth[s]);
+  t->lvl_nsys.assign(t->dmax + 1, 0);
+  t->h_lvl_off.assign(t->dmax + 2, 0);
+  std::vector<int32_t> lvl;
+  std::vector<int64_t> soff(nsys);
+  for (int d = 0; d <= t->dmax; d++) {
+    t->h_lvl_off[d] = (int64_t)lvl.size();
+    for (int64_t s = 0; s < nsys; s++)
+      if (depth[s] >= d) {
+        lvl.push_back((int32_t)s);
+        t->lvl_nsys[d]++;
+      }
+  }
+  t->h_lvl_off[t->dmax + 1] = (int64_t)lvl.size();
+  for (int64_t s = 0; s < nsys; s++) soff[s] = info[s].slot_off;
+  t->slot_node = dalloc_t<int32_t>(c, (size_t)slots);
+  if (!t->slot_node) return BH_ENOMEM;
+  if (nsys > 1) {
+    t->d_lvl_sys = dalloc_t<int32_t>(c, lvl.size());
+    t->d_slot_off = dalloc_t<int64_t>(c, (size_t)nsys);
+    if (!t->d_lvl_sys || !t->d_slot_off) return BH_ENOMEM;
+    BH_HIP(hipMemcpyAsync(t->d_lvl_sys, lvl.data(), lvl.size() * 4, hipMemcpyHostToDevice, c->stream));
+    BH_HIP(hipMemcpyAsync(t->d_slot_off, soff.data(), nsys * 8, hipMemcpyHostToDevice, c->stream));
   }
   Scratch sc(c);
   TopoTables* d_tb = sc.get<TopoTables>(nsys);
@@ -215,7 +255,7 @@ int topology(bh_ctx* c, bh_tree* t) {
   }
   const int nb = (int)std::min<int64_t>(cdiv(slots, 256), 65536);
   hipLaunchKernelGGL(k_topology, dim3(nb), dim3(256), 0, c->stream, nsys, L, slots, d_info, d_tb, t->start,
-                     t->count, t->first_child, t->parent, t->skip, t->leaf_order);
+                     t->count, t->first_child, t->parent, t->skip, t->leaf_order, t->slot_node);
   BH_LAUNCH_CHECK();
   // the host tables of a forest must outlive the asynchronous copies
   if (nsys > 1) BH_HIP(hipStreamSynchronize(c->stream));
@@ -306,20 +346,12 @@ __device__ __forceinline__ bool arrive(uint32_t* flags, int node) {
   return old != 0;
 }
 
-// One thread per leaf, leaves in left-to-right (memory) order.  Leaves:
-// tight box (build.py:232-242), mass and centre of mass in NumPy order
-// (build.py:258-272); then climb: the second thread to reach a parent
-// combines its two children (build.py:274-295, or refit.py:152-192 when
-// REFIT, which keeps centre/half_size and adds only children with mass > 0).
+// Leaf v: tight box (build.py:232-242), mass and centre of mass in NumPy
+// order (build.py:258-272); REFIT keeps the box and centre.
 template <int DIM, bool REFIT>
-__global__ __launch_bounds__(256) void k_leaf_up(
-    int64_t nleaves, const int32_t* __restrict__ leaf_order, const int32_t* __restrict__ start,
-    const int32_t* __restrict__ count, const int32_t* __restrict__ first_child,
-    const int32_t* __restrict__ parent, const double4* __restrict__ spart, double* center,
-    double* half, double* mass, double* com, double* bmin, double* bmax, uint32_t* flags) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= nleaves) return;
-  const int v = leaf_order[t];
+__device__ __forceinline__ void up_leaf(int v, const int32_t* __restrict__ start, const int32_t* __restrict__ count,
+                                        const double4* __restrict__ spart, double* center, double* half,
+                                        double* mass, double* com, double* bmin, double* bmax) {
   const int64_t s = start[v], c = count[v];
   const double4* p = spart + s;
   double ctr[DIM];
@@ -369,69 +401,120 @@ __global__ __launch_bounds__(256) void k_leaf_up(
 #pragma unroll
     for (int d = 0; d < DIM; d++) com[v * DIM + d] = ctr[d];
   }
+}
 
+// Internal node from its two children (build.py:274-295, or
+// refit.py:152-192 when REFIT, which keeps centre/half_size and adds only
+// children with mass > 0).
+template <int DIM, bool REFIT>
+__device__ __forceinline__ void up_node(int node, const int32_t* __restrict__ first_child, double* center,
+                                        double* half, double* mass, double* com, double* bmin, double* bmax) {
+  const int a = first_child[node], b = a + 1;
+  double cc[DIM];
+  if (!REFIT) {
+    double em = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      const double lo = np_min(bmin[a * DIM + d], bmin[b * DIM + d]);
+      const double hi = np_max(bmax[a * DIM + d], bmax[b * DIM + d]);
+      bmin[node * DIM + d] = lo;
+      bmax[node * DIM + d] = hi;
+      cc[d] = (lo + hi) / 2;
+      center[node * DIM + d] = cc[d];
+      em = d == 0 ? hi - lo : np_max(em, hi - lo);
+    }
+    half[node] = em / 2;
+    const double ma = mass[a], mb = mass[b];
+    double tsum = -0.0;
+    tsum += ma;
+    tsum += mb;
+    const double tot = 0.0 + tsum;
+    mass[node] = tot;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      double sd = 0.0;
+      const double wa = ma * com[a * DIM + d];
+      sd = sd + wa;
+      const double wb = mb * com[b * DIM + d];
+      sd = sd + wb;
+      com[node * DIM + d] = tot > 0 ? sd / tot : cc[d];
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) cc[d] = center[node * DIM + d];
+    const double ma = mass[a], mb = mass[b];
+    double tot = 0.0, wc[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) wc[d] = 0.0;
+    if (ma > 0) {
+      tot += ma;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const double w = ma * com[a * DIM + d];
+        wc[d] = wc[d] + w;
+      }
+    }
+    if (mb > 0) {
+      tot += mb;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const double w = mb * com[b * DIM + d];
+        wc[d] = wc[d] + w;
+      }
+    }
+    mass[node] = tot;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) com[node * DIM + d] = tot > 0 ? wc[d] / tot : cc[d];
+  }
+}
+
+// Trees without depth tables (imported): one thread per leaf, leaves in
+// left-to-right (memory) order; after its leaf a thread climbs, and the
+// second thread to reach a parent combines its two children.
+template <int DIM, bool REFIT>
+__global__ __launch_bounds__(256) void k_leaf_up(
+    int64_t nleaves, const int32_t* __restrict__ leaf_order, const int32_t* __restrict__ start,
+    const int32_t* __restrict__ count, const int32_t* __restrict__ first_child,
+    const int32_t* __restrict__ parent, const double4* __restrict__ spart, double* center,
+    double* half, double* mass, double* com, double* bmin, double* bmax, uint32_t* flags) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nleaves) return;
+  const int v = leaf_order[t];
+  up_leaf<DIM, REFIT>(v, start, count, spart, center, half, mass, com, bmin, bmax);
   int node = parent[v];
   while (node >= 0) {
     if (!arrive(flags, node)) return;
-    const int a = first_child[node], b = a + 1;
-    double cc[DIM];
-    if (!REFIT) {
-      double em = 0.0;
-#pragma unroll
-      for (int d = 0; d < DIM; d++) {
-        const double lo = np_min(bmin[a * DIM + d], bmin[b * DIM + d]);
-        const double hi = np_max(bmax[a * DIM + d], bmax[b * DIM + d]);
-        bmin[node * DIM + d] = lo;
-        bmax[node * DIM + d] = hi;
-        cc[d] = (lo + hi) / 2;
-        center[node * DIM + d] = cc[d];
-        em = d == 0 ? hi - lo : np_max(em, hi - lo);
-      }
-      half[node] = em / 2;
-      const double ma = mass[a], mb = mass[b];
-      double tsum = -0.0;
-      tsum += ma;
-      tsum += mb;
-      const double tot = 0.0 + tsum;
-      mass[node] = tot;
-#pragma unroll
-      for (int d = 0; d < DIM; d++) {
-        double sd = 0.0;
-        const double wa = ma * com[a * DIM + d];
-        sd = sd + wa;
-        const double wb = mb * com[b * DIM + d];
-        sd = sd + wb;
-        com[node * DIM + d] = tot > 0 ? sd / tot : cc[d];
-      }
-    } else {
-#pragma unroll
-      for (int d = 0; d < DIM; d++) cc[d] = center[node * DIM + d];
-      const double ma = mass[a], mb = mass[b];
-      double tot = 0.0, wc[DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; d++) wc[d] = 0.0;
-      if (ma > 0) {
-        tot += ma;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-          const double w = ma * com[a * DIM + d];
-          wc[d] = wc[d] + w;
-        }
-      }
-      if (mb > 0) {
-        tot += mb;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-          const double w = mb * com[b * DIM + d];
-          wc[d] = wc[d] + w;
-        }
-      }
-      mass[node] = tot;
-#pragma unroll
-      for (int d = 0; d < DIM; d++) com[node * DIM + d] = tot > 0 ? wc[d] / tot : cc[d];
-    }
+    up_node<DIM, REFIT>(node, first_child, center, half, mass, com, bmin, bmax);
     node = parent[node];
   }
+}
+
+// The node of slot `slot` of depth d among the deep-enough systems:
+// nthreads = 2^d x their number; lvl_sys lists them (NULL: one tree);
+// slot_off gives each system's first slot (NULL: one tree); slot_node maps a
+// slot to its node (-1: below a leaf).
+__device__ __forceinline__ int level_node(int d, int64_t t, const int32_t* __restrict__ lvl_sys,
+                                          const int64_t* __restrict__ slot_off,
+                                          const int32_t* __restrict__ slot_node) {
+  const int64_t sys = lvl_sys ? lvl_sys[t >> d] : 0;
+  const int64_t base = slot_off ? slot_off[sys] : 0;
+  return slot_node[base + (1ll << d) - 1 + (t & ((1ll << d) - 1))];
+}
+
+// Trees built here: one launch per depth, deepest first, so a node's
+// children are complete when it is combined (no atomics, no fences).
+template <int DIM, bool REFIT>
+__global__ __launch_bounds__(256) void k_up_level(
+    int d, int64_t nthreads, const int32_t* __restrict__ lvl_sys, const int64_t* __restrict__ slot_off,
+    const int32_t* __restrict__ slot_node, const int32_t* __restrict__ start, const int32_t* __restrict__ count,
+    const int32_t* __restrict__ first_child, const double4* __restrict__ spart, double* center, double* half,
+    double* mass, double* com, double* bmin, double* bmax) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nthreads) return;
+  const int v = level_node(d, t, lvl_sys, slot_off, slot_node);
+  if (v < 0) return;
+  if (first_child[v] < 0) up_leaf<DIM, REFIT>(v, start, count, spart, center, half, mass, com, bmin, bmax);
+  else up_node<DIM, REFIT>(v, first_child, center, half, mass, com, bmin, bmax);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
@@ -447,6 +530,25 @@ int fill_u32(bh_ctx* c, uint32_t* p, uint32_t v, int64_t n) {
 }
 
 int upward(bh_ctx* c, bh_tree* t, int refit) {
+  if (t->slot_node) {
+    for (int d = t->dmax; d >= 0; d--) {
+      const int64_t nthr = t->lvl_nsys[d] << d;
+      const int32_t* ls = t->d_lvl_sys ? t->d_lvl_sys + t->h_lvl_off[d] : nullptr;
+      const unsigned nb = (unsigned)cdiv(nthr, 256);
+#define BH_UPL(D, R)                                                                                        \
+  hipLaunchKernelGGL((k_up_level<D, R>), dim3(nb), dim3(256), 0, c->stream, d, nthr, ls, t->d_slot_off,     \
+                     t->slot_node, t->start, t->count, t->first_child, t->spart, t->center, t->half, t->mass, \
+                     t->com, t->bmin, t->bmax)
+      if (t->dim == 3) {
+        if (refit) BH_UPL(3, true); else BH_UPL(3, false);
+      } else {
+        if (refit) BH_UPL(2, true); else BH_UPL(2, false);
+      }
+#undef BH_UPL
+      BH_LAUNCH_CHECK();
+    }
+    return BH_OK;
+  }
   Scratch s(c);
   uint32_t* flags = s.get<uint32_t>((size_t)t->M);
   if (!flags) return BH_ENOMEM;

@@ -153,6 +153,14 @@ struct bh_tree {
   int32_t* parent = nullptr;
   int32_t* skip = nullptr;
   int32_t* leaf_order = nullptr;  // leaves left to right
+  // depth tables of a built tree (NULL slot_node: imported or cloned): slot
+  // (system, depth d, position) -> node or -1; per depth the number of
+  // systems deep enough, their ids (forest) and each system's first slot
+  int32_t* slot_node = nullptr;
+  int dmax = 0;
+  std::vector<int64_t> lvl_nsys, h_lvl_off;
+  int32_t* d_lvl_sys = nullptr;
+  int64_t* d_slot_off = nullptr;
   // geometry and monopoles in TreeData layout
   double* center = nullptr;   // M*dim
   double* half = nullptr;     // M

@@ -73,9 +73,48 @@ __global__ __launch_bounds__(256) void k_src32(int64_t nleaves, const int32_t* _
   }
 }
 
-// Second moments Q = sum m (x - com)(x - com)^T, FP64, by the same
-// atomic-flag climb as the build's upward pass: leaves from their particles,
-// internal nodes by the parallel-axis shift of their children's moments.
+// Second moments Q = sum m (x - com)(x - com)^T, FP64: leaves from their
+// particles, internal nodes by the parallel-axis shift of their children's
+// moments.
+__device__ __forceinline__ void quad_leaf(int v, const int32_t* __restrict__ start, const int32_t* __restrict__ count,
+                                          const double4* spart, const double* com, int dim, double* quad) {
+  const double cx = com[v * dim], cy = com[v * dim + 1], cz = dim == 3 ? com[v * dim + 2] : 0.0;
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  const int64_t s = start[v], c = count[v];
+  for (int64_t j = s; j < s + c; j++) {
+    const double4 p = spart[j];
+    const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+    q[0] += p.w * dx * dx;
+    q[1] += p.w * dy * dy;
+    q[2] += p.w * dz * dz;
+    q[3] += p.w * dx * dy;
+    q[4] += p.w * dx * dz;
+    q[5] += p.w * dy * dz;
+  }
+  for (int k = 0; k < 6; k++) quad[(int64_t)v * 6 + k] = q[k];
+}
+
+__device__ __forceinline__ void quad_node(int node, const int32_t* __restrict__ first_child, const double* com,
+                                          int dim, const double* mass, double* quad) {
+  const double cx = com[node * dim], cy = com[node * dim + 1], cz = dim == 3 ? com[node * dim + 2] : 0.0;
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  const int a = first_child[node];
+  for (int ch = a; ch < a + 2; ch++) {
+    const double m = mass[ch];
+    const double dx = com[ch * dim] - cx, dy = com[ch * dim + 1] - cy, dz = dim == 3 ? com[ch * dim + 2] - cz : 0.0;
+    const double* qc = quad + (int64_t)ch * 6;
+    q[0] += qc[0] + m * dx * dx;
+    q[1] += qc[1] + m * dy * dy;
+    q[2] += qc[2] + m * dz * dz;
+    q[3] += qc[3] + m * dx * dy;
+    q[4] += qc[4] + m * dx * dz;
+    q[5] += qc[5] + m * dy * dz;
+  }
+  for (int k = 0; k < 6; k++) quad[(int64_t)node * 6 + k] = q[k];
+}
+
+// Trees without depth tables (imported): the atomic-flag climb of the
+// build's upward pass.
 __device__ __forceinline__ bool arrive_q(uint32_t* flags, int node) {
   return __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
@@ -90,42 +129,32 @@ __global__ __launch_bounds__(256) void k_quad_up(int64_t nleaves, const int32_t*
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nleaves) return;
   const int v = leaf_order[t];
-  {
-    const double cx = com[v * dim], cy = com[v * dim + 1], cz = dim == 3 ? com[v * dim + 2] : 0.0;
-    double q[6] = {0, 0, 0, 0, 0, 0};
-    const int64_t s = start[v], c = count[v];
-    for (int64_t j = s; j < s + c; j++) {
-      const double4 p = spart[j];
-      const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
-      q[0] += p.w * dx * dx;
-      q[1] += p.w * dy * dy;
-      q[2] += p.w * dz * dz;
-      q[3] += p.w * dx * dy;
-      q[4] += p.w * dx * dz;
-      q[5] += p.w * dy * dz;
-    }
-    for (int k = 0; k < 6; k++) quad[(int64_t)v * 6 + k] = q[k];
-  }
+  quad_leaf(v, start, count, spart, com, dim, quad);
   int node = parent[v];
   while (node >= 0) {
     if (!arrive_q(flags, node)) return;
-    const double cx = com[node * dim], cy = com[node * dim + 1], cz = dim == 3 ? com[node * dim + 2] : 0.0;
-    double q[6] = {0, 0, 0, 0, 0, 0};
-    const int a = first_child[node];
-    for (int ch = a; ch < a + 2; ch++) {
-      const double m = mass[ch];
-      const double dx = com[ch * dim] - cx, dy = com[ch * dim + 1] - cy, dz = dim == 3 ? com[ch * dim + 2] - cz : 0.0;
-      const double* qc = quad + (int64_t)ch * 6;
-      q[0] += qc[0] + m * dx * dx;
-      q[1] += qc[1] + m * dy * dy;
-      q[2] += qc[2] + m * dz * dz;
-      q[3] += qc[3] + m * dx * dy;
-      q[4] += qc[4] + m * dx * dz;
-      q[5] += qc[5] + m * dy * dz;
-    }
-    for (int k = 0; k < 6; k++) quad[(int64_t)node * 6 + k] = q[k];
+    quad_node(node, first_child, com, dim, mass, quad);
     node = parent[node];
   }
+}
+
+// Trees built here: one launch per depth, deepest first (the slot tables of
+// the build's level-by-level upward pass).
+__global__ __launch_bounds__(256) void k_quad_level(int d, int64_t nthreads, const int32_t* __restrict__ lvl_sys,
+                                                    const int64_t* __restrict__ slot_off,
+                                                    const int32_t* __restrict__ slot_node,
+                                                    const int32_t* __restrict__ start,
+                                                    const int32_t* __restrict__ count,
+                                                    const int32_t* __restrict__ first_child, const double4* spart,
+                                                    const double* com, int dim, const double* mass, double* quad) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nthreads) return;
+  const int64_t sys = lvl_sys ? lvl_sys[t >> d] : 0;
+  const int64_t base = slot_off ? slot_off[sys] : 0;
+  const int v = slot_node[base + (1ll << d) - 1 + (t & ((1ll << d) - 1))];
+  if (v < 0) return;
+  if (first_child[v] < 0) quad_leaf(v, start, count, spart, com, dim, quad);
+  else quad_node(v, first_child, com, dim, mass, quad);
 }
 
 // Accepted-cell payload of every node in the layout the cell list uses:
@@ -152,7 +181,18 @@ int f32_prepare(bh_ctx* c, bh_tree* t) {
     hipLaunchKernelGGL(k_src32<2>, dim3(nb), dim3(256), 0, c->stream, t->nleaves, t->leaf_order, t->start,
                        t->count, t->com, t->spart, t->spart32, t->pleaf);
   BH_LAUNCH_CHECK();
-  if (t->binary) {
+  if (t->binary && t->slot_node) {
+    for (int d = t->dmax; d >= 0; d--) {
+      const int64_t nthr = t->lvl_nsys[d] << d;
+      const int32_t* ls = t->d_lvl_sys ? t->d_lvl_sys + t->h_lvl_off[d] : nullptr;
+      hipLaunchKernelGGL(k_quad_level, dim3((unsigned)cdiv(nthr, 256)), dim3(256), 0, c->stream, d, nthr, ls,
+                         t->d_slot_off, t->slot_node, t->start, t->count, t->first_child, t->spart, t->com, t->dim,
+                         t->mass, t->quad);
+    }
+    hipLaunchKernelGGL(k_quad32, dim3((unsigned)cdiv(t->M, 256)), dim3(256), 0, c->stream, t->M, t->quad, t->quad32);
+    BH_LAUNCH_CHECK();
+    t->quad_ready = true;
+  } else if (t->binary) {
     Scratch s(c);
     uint32_t* flags = s.get<uint32_t>((size_t)t->M);
     if (!flags) return BH_ENOMEM;
