@@ -467,14 +467,16 @@ void launch_walk(bh_ctx* c, const bh_tree* t, dim3 grid, size_t lds, int gs, con
 }
 
 // ---------------------------------------------------------------- eval
-template <bool ZEPS>
+// UNI: every source has the same mass, applied to the tile's sum instead
+// (one multiply less per pair).
+template <bool ZEPS, bool UNI>
 __device__ __forceinline__ void p2p(float4 s, float tx, float ty, float tz, float eps2, float& px, float& py,
                                     float& pz) {
   const float dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
   float ri = __builtin_amdgcn_rsqf(r2);
   if (ZEPS) ri = r2 > 0.f ? ri : 0.f;
-  const float mg = s.w * (ri * ri * ri);
+  const float mg = UNI ? ri * ri * ri : s.w * (ri * ri * ri);
   px = fmaf(mg, dx, px);
   py = fmaf(mg, dy, py);
   pz = fmaf(mg, dz, pz);
@@ -519,20 +521,22 @@ __device__ __forceinline__ void kahan_add(float v, float& s, float& c) {
   s = t;
 }
 
+// The tile's sums p (times the tile's common factor f) into the running
+// sums s (compensated by c when KAHAN).
 template <int TPL, bool KAHAN>
 __device__ __forceinline__ void flush(float (&px)[TPL], float (&py)[TPL], float (&pz)[TPL], float (&sx)[TPL],
                                       float (&sy)[TPL], float (&sz)[TPL], float (&cx)[TPL], float (&cy)[TPL],
-                                      float (&cz)[TPL]) {
+                                      float (&cz)[TPL], float f = 1.f) {
 #pragma unroll
   for (int q = 0; q < TPL; q++) {
     if (KAHAN) {
-      kahan_add(px[q], sx[q], cx[q]);
-      kahan_add(py[q], sy[q], cy[q]);
-      kahan_add(pz[q], sz[q], cz[q]);
+      kahan_add(f * px[q], sx[q], cx[q]);
+      kahan_add(f * py[q], sy[q], cy[q]);
+      kahan_add(f * pz[q], sz[q], cz[q]);
     } else {
-      sx[q] += px[q];
-      sy[q] += py[q];
-      sz[q] += pz[q];
+      sx[q] += f * px[q];
+      sy[q] += f * py[q];
+      sz[q] += f * pz[q];
     }
     px[q] = py[q] = pz[q] = 0.f;
   }
@@ -738,16 +742,30 @@ __device__ __forceinline__ void eval_item(
                                           src_cur.w)
                             : make_float4(1e15f, 1e15f, 1e15f, 0.f);
       wave_sync();
+      // a tile whose sources share one finite mass takes the loop without the
+      // per-pair mass product; its sum is scaled by that mass once
+      const float um = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(src_cur.w)));
+      const bool uni = __ballot(v_cur && src_cur.w != um) == 0 && __builtin_isfinite(um);
       const int nslots = n_cur;
       const bool more = !(tp >= wtot && w + nw >= ke);
       if (more) n_cur = map_tile(src_cur, sh_cur, v_cur);  // the next tile's loads are in flight
+      if (uni) {
 #pragma unroll 8
-      for (int k = 0; k < nslots; k++) {
-        const float4 v = tile[0][k];
+        for (int k = 0; k < nslots; k++) {
+          const float3 v3 = *reinterpret_cast<const float3*>(&tile[0][k]);  // the mass is not read
+          const float4 v = make_float4(v3.x, v3.y, v3.z, 0.f);
 #pragma unroll
-        for (int q = 0; q < TPL; q++) p2p<ZEPS>(v, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
+          for (int q = 0; q < TPL; q++) p2p<ZEPS, true>(v, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
+        }
+      } else {
+#pragma unroll 8
+        for (int k = 0; k < nslots; k++) {
+          const float4 v = tile[0][k];
+#pragma unroll
+          for (int q = 0; q < TPL; q++) p2p<ZEPS, false>(v, tx[q], ty[q], tz[q], eps2, px[q], py[q], pz[q]);
+        }
       }
-      flush<TPL, KAHAN>(px, py, pz, sx, sy, sz, cx, cy, cz);
+      flush<TPL, KAHAN>(px, py, pz, sx, sy, sz, cx, cy, cz, uni ? um : 1.f);
       if (!more) break;
     }
   }
@@ -1078,8 +1096,8 @@ template <int TPL, int CH>
 static void launch_eval(bh_ctx* c, int64_t g0, int64_t ng, const int32_t* ioff, bool zeps, bool self, bool kahan,
                         bool quad, int64_t m, int gs, const double4* tpos, const bh_tree* t, const GroupBox* boxes,
                         const int64_t* coff, const CellEnt* clist, const int64_t* koff, const ChunkEnt* klist,
-                        const int64_t* ccnt, const int64_t* kcnt, float eps2, const int32_t* gt0, const uint32_t* err, uint32_t* next, double* part,
-                        int64_t items_bound) {
+                        const int64_t* ccnt, const int64_t* kcnt, float eps2, const int32_t* gt0,
+                        const uint32_t* err, uint32_t* next, double* part, int64_t items_bound) {
 #define BH_EV(Z, S, K, Q)                                                                                          \
   {                                                                                                                \
     const void* kern = reinterpret_cast<const void*>(&k_group_eval<TPL, CH, Z, S, K, Q>);                         \

@@ -40,6 +40,28 @@ def test_theta_zero_is_fp32_direct_sum(native):
     assert rms < 2e-6 and mx < 5e-5, (rms, mx)
 
 
+@pytest.mark.parametrize("kind", ["random", "two_masses"])
+def test_unequal_masses_theta_zero(native, kind):
+    """Source tiles whose masses differ multiply each pair by its mass; tiles
+    of one common mass apply it to the tile's sum.  Random masses take the
+    first path everywhere, a disk (mass 1/6) + halo (5/6) mix takes both;
+    either way an FP32 direct sum: random masses bounded as for equal ones;
+    the disk's in-plane forces cancel more (looser bounds, still orders of
+    magnitude below a wrong mass factor)."""
+    if kind == "random":
+        x, _ = workloads.plummer(20_000, seed=4)
+        m = np.random.default_rng(4).uniform(0.5, 1.5, 20_000) / 20_000
+        bound = (2e-6, 5e-5)
+    else:
+        x, m = workloads.disk_halo(10_000, 10_000, seed=5)
+        bound = (1e-5, 1e-3)
+    dt = native.DeviceTree.build(x, m, 32)
+    a = dt.accel_f32(None, 0.0, 0.01, 1.0)
+    d = native.direct(x, m, softening=0.01)
+    rms, mx = rel_err(a, d)
+    assert rms < bound[0] and mx < bound[1], (kind, rms, mx)
+
+
 @pytest.mark.parametrize("gs,kahan,multipole", [(64, 1, 2), (128, 1, 2), (64, 0, 2), (64, 1, 1)])
 def test_theta_half_accuracy(native, gs, kahan, multipole):
     """rms relative error against FP64 direct summation: with quadrupole
