@@ -26,6 +26,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -145,6 +146,20 @@ def roofline_f32(prof, steps, stats):
             "launches_per_step": launches / max(steps, 1), "kernel_ms_per_step": kstep,
             "flops_per_step": flops, "cell_interactions_per_step": int(stats[0]),
             "pair_interactions_per_step": int(stats[1]), "interactions_per_launch": int(stats[0] + stats[1])}
+
+
+def pair_loop_ceiling():
+    """Pairs/s of the evaluation's leaf-pair loop alone on this device
+    (bhut_amd/pair_ceiling, built with the library), or None."""
+    exe = os.path.join(ROOT, "bhut_amd", "pair_ceiling")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=120, check=True)
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception as e:  # reported, not fatal: the bench line stands without it
+        print(f"bench: pair-loop ceiling not measured ({e})", file=sys.stderr)
+        return None
 
 
 def cpu_baseline(x, m, theta, eps, leaf, budget_s):
@@ -282,6 +297,16 @@ def workload_plummer(args, env):
     line = base_line(args, N / (ms * 1e-3), ms, "fp32" if head == "fp32" else "fp64",
                      {"workload": "plummer_self_accelerations", "n": N, "theta": theta, "softening": eps,
                       "leaf_size": leaf, "G": G, "seed": 1, "step": "tree build + self accelerations", "form": head})
+    if head == "fp32" and roof.get("achieved"):
+        # the same loop without lists, windows or cells: what this
+        # instruction mix can reach on the device (equal masses here)
+        ceil = pair_loop_ceiling()
+        if ceil:
+            tf = ceil["uniform"] * FLOPS_PER_INTERACTION / 1e12
+            roof["loop_ceiling"] = {"pairs_per_s": ceil["uniform"], "tflops": tf, "frac": roof["achieved"] / tf,
+                                    "mass_pairs_per_s": ceil["mass"], "no_rsq_pairs_per_s": ceil["no_rsq"]}
+            print(f"bench: pair-loop ceiling {tf:.1f} TFLOP/s; evaluation at {roof['achieved'] / tf:.1%} of it",
+                  file=sys.stderr)
     line["roofline"] = roof
     line["accuracy"] = acc_err
     line["interactions_per_s"] = inter / (ms * 1e-3)
