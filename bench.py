@@ -184,6 +184,17 @@ def cpu_baseline(x, m, theta, eps, leaf, budget_s):
         chunk = min(chunk * 2, 200_000)
     per_target = t_acc / done
     t_total = t_build + per_target * n
+    # the serial loop of the reference (one target after another on one
+    # core), on a smaller sample of the same targets: about a tenth of the
+    # budget
+    done1, t_acc1 = 0, 0.0
+    while done1 < n and t_acc1 < 0.1 * budget_s:
+        sel = order[done1:done1 + 500]
+        t0 = time.perf_counter()
+        O.accelerations(tree, x, m, x[sel], softening=eps, theta=theta, nthreads=1)
+        t_acc1 += time.perf_counter() - t0
+        done1 += len(sel)
+    t_total1 = t_build + t_acc1 / done1 * n
     cpu = "unknown"
     try:
         cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
@@ -197,6 +208,9 @@ def cpu_baseline(x, m, theta, eps, leaf, budget_s):
         "sample": f"oracle build of all {n} particles ({t_build:.2f} s, 1 thread) + accelerations of {done} "
                   f"random targets ({t_acc:.2f} s, {threads} OpenMP threads), extrapolated to {n} targets; "
                   f"host CPU {cpu}",
+        "single_thread": {"value": n / t_total1, "unit": "particle-accels/s", "cores": 1,
+                          "sample": f"the same build + {done1} random targets ({t_acc1:.2f} s, 1 thread), "
+                                    f"extrapolated to {n} targets"},
     }
 
 
