@@ -175,6 +175,30 @@ def test_refit_matches_oracle_and_rebuild_rule(native):
     bits_equal(t._tree.half_size, ob.half_size, "rebuilt half_size")
 
 
+def test_refit_tree_on_clone_and_import(native):
+    """refit_tree (refit.py:17-109) on a tree built here refits a clone, and
+    on host-only arrays an imported tree: both carry no depth tables and go
+    up by the atomic-flag climb instead of one launch per depth.  Either way
+    the oracle's bits, with unequal masses (NumPy's pairwise leaf sums)."""
+    from bhut_amd.tree.build import build_tree
+    from bhut_amd.tree.node import TreeData
+    from bhut_amd.tree.refit import refit_tree
+    rng = np.random.default_rng(12)
+    x = plummer(20_000, 12)
+    m = rng.uniform(0.5, 1.5, len(x)) / len(x)
+    x2 = x + 0.001 * rng.standard_normal(x.shape)
+    ot = O.refit(O.build(x, m, 32), x2, m)
+    tree = build_tree(x, m, dim=3, leaf_size=32)
+    host = TreeData(**tree.arrays(), dim=3, leaf_size=32)
+    assert host.device_tree is None
+    for name, src in (("clone", tree), ("import", host)):
+        r = refit_tree(src, x2, m)
+        for f in ("mass", "com", "center", "half_size"):
+            bits_equal(getattr(r, f), getattr(ot, f), f"{name} {f}")
+    # the source tree is untouched by the refit of its clone
+    bits_equal(tree.device_tree.export()["mass"], O.build(x, m, 32).mass, "source mass")
+
+
 def test_edge_cases(native):
     import bhut_amd
     # single particle: zero acceleration
