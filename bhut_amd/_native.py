@@ -43,7 +43,7 @@ _SIGS = {
     "bh_ctx_profile_summary": (ctypes.c_int, [vp, ctypes.c_char_p, i64]),
     "bh_tree_build": (ctypes.c_int, [vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
     "bh_tree_build_dev": (ctypes.c_int, [vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
-    "bh_tree_import": (ctypes.c_int, [vp, i64, i64, i32, i32] + [vp] * 13 + [ctypes.POINTER(vp)]),
+    "bh_tree_import": (ctypes.c_int, [vp, i64, i64, i32, i32] + [vp] * 11 + [ctypes.POINTER(vp)]),
     "bh_tree_clone": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "bh_forest_build": (ctypes.c_int, [vp, vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),
     "bh_forest_build_dev": (ctypes.c_int, [vp, vp, vp, vp, i64, i32, i32, ctypes.POINTER(vp)]),

@@ -35,6 +35,26 @@ def test_library_exports_every_declared_symbol():
     assert set(_native._SIGS) == set(syms)
 
 
+def declared_arities():
+    """Parameter count of every function the header declares."""
+    src = open(os.path.join(ROOT, "include", "bhut_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b(bh_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", src):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_table_matches_header_arity():
+    """Every ctypes binding takes as many arguments as its C declaration (a
+    mismatch is only noticed when the binding is first called)."""
+    ar = declared_arities()
+    assert set(ar) == set(_native._SIGS)
+    for name, (_, argtypes) in _native._SIGS.items():
+        assert len(argtypes) == ar[name], (name, len(argtypes), ar[name])
+
+
 def test_library_reports_no_device_cleanly():
     if _native.device_count() > 0:
         pytest.skip("a device is visible")
