@@ -199,6 +199,33 @@ def test_refit_tree_on_clone_and_import(native):
     bits_equal(tree.device_tree.export()["mass"], O.build(x, m, 32).mass, "source mass")
 
 
+def test_traversal_with_other_source_arrays(native):
+    """barnes_hut_accelerations(_targets) (bh.py:428-482, 30-223) walking a
+    TreeData with source arrays other than the ones it was built from (the
+    tree is imported with those arrays), and with host-only TreeData: the
+    oracle's accelerations to 1e-12, lists exact."""
+    from bhut_amd.traverse.bh import (barnes_hut_accelerations, barnes_hut_accelerations_targets,
+                                      interaction_lists)
+    from bhut_amd.tree.build import build_tree
+    from bhut_amd.tree.node import TreeData
+    rng = np.random.default_rng(13)
+    x = plummer(8_000, 13)
+    m = rng.uniform(0.5, 1.5, len(x))
+    ot = O.build(x, m, 16)
+    tree = build_tree(x, m, dim=3, leaf_size=16)
+    host = TreeData(**tree.arrays(), dim=3, leaf_size=16)
+    m2 = m * 2.0  # other source masses on the same tree
+    tg = rng.uniform(-3, 3, (500, 3))
+    for name, tr in (("device", tree), ("host", host)):
+        acc_close(barnes_hut_accelerations(tr, x.copy(), m2, 0.01, 0.5),
+                  O.accelerations(ot, x, m2, softening=0.01, theta=0.5))
+        acc_close(barnes_hut_accelerations_targets(tr, x.copy(), m2, tg, 0.01, 0.5),
+                  O.accelerations(ot, x, m2, tg, softening=0.01, theta=0.5))
+        ptr, ids = interaction_lists(tr, x.copy(), m, 0.5)
+        for i in (0, 17, 4000):
+            np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], O.target_list(ot, x[i], 0.5), err_msg=name)
+
+
 def test_edge_cases(native):
     import bhut_amd
     # single particle: zero acceleration
