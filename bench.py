@@ -582,8 +582,8 @@ def workload_copy(args, env):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the target-sharded multi-GPU design is "
-              "described in DESIGN.md and not measured here", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path is measured on one GPU's "
+              "share; sharding targets over several GPUs is a design, not measured here", file=sys.stderr)
         sys.exit(2)
     env = Env()
     if args.list_budget_gib > 0:

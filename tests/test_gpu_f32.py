@@ -96,10 +96,13 @@ def test_external_targets(native):
     assert rel_err(a, d)[0] <= rel_err(a64, d)[0] * 1.05
 
 
-def test_two_dimensional_and_zero_softening(native):
+@pytest.mark.parametrize("masses", ["random", "equal"])
+def test_two_dimensional_and_zero_softening(native, masses):
+    """2-D, softened and unsoftened (the pair with r = 0 adds nothing), with
+    per-pair masses and with the equal-mass tiles."""
     rng = np.random.default_rng(5)
     x = rng.standard_normal((8000, 2))
-    m = rng.uniform(0.5, 1.5, 8000)
+    m = rng.uniform(0.5, 1.5, 8000) if masses == "random" else np.full(8000, 1.0 / 8000)
     dt = native.DeviceTree.build(x, m, 16)
     for eps in (0.01, 0.0):
         a = dt.accel_f32(None, 0.0, eps, 1.0)
