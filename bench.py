@@ -358,6 +358,20 @@ def workload_plummer(args, env):
             log(f"bench: query {form} {qms:.3f} ms for {M} targets; {qres[form]['accuracy']}")
         line["query"] = {"targets": M, "distribution": "uniform [-5,5]^3, seed 11", "forms": qres}
 
+    if head == "fp32":
+        # the drop-in call on host NumPy arrays: positions and masses copied
+        # in and accelerations copied out over PCIe around the same step
+        import bhut_amd
+        kw = dict(theta=theta, softening=eps, G=G, leaf_size=leaf, precision="fp32")
+        bhut_amd.accelerations(x, m, **kw)
+        reps, t0 = 3, time.perf_counter()
+        for _ in range(reps):
+            bhut_amd.accelerations(x, m, **kw)
+        hms = (time.perf_counter() - t0) / reps * 1e3
+        line["host_api"] = {"call": "bhut_amd.accelerations(x, m, precision='fp32') on host arrays",
+                            "ms": hms, "particle_accels_per_s": N / (hms * 1e-3)}
+        log(f"bench: host-array call {hms:.2f} ms ({N / (hms * 1e-3):.3e} particle-accels/s incl. PCIe)")
+
     if not args.no_cpu:
         log(f"bench: CPU baseline (oracle port), budget {args.cpu_seconds:.0f} s")
         line["cpu_baseline"] = cpu_baseline(x, m, theta, eps, leaf, args.cpu_seconds)
