@@ -83,6 +83,22 @@ def test_theta_half_accuracy(native, gs, kahan, multipole):
     assert st[0] > 0 and st[1] > 0 and st[2] == -(-n // gs)
 
 
+def test_theta_half_accuracy_two_masses(native):
+    """The disk (mass 1/6) + halo (5/6) mix at theta = 0.5, where tiles of
+    one mass and mixed tiles alternate: rms relative error against FP64
+    direct summation below 1e-3 and no worse than the plain per-target walk."""
+    x, m = workloads.disk_halo(100_000, 100_000, seed=8)
+    n = len(x)
+    dt = native.DeviceTree.build(x, m, 16)
+    a32 = dt.accel_f32(None, 0.5, 0.01, 1.0)
+    a64 = dt.accel_self(0.5, 0.01, 1.0)
+    sel = np.random.default_rng(8).choice(n, 2000, replace=False)
+    d = native.direct(x, m, x[sel], softening=0.01)
+    rms32, _ = rel_err(a32[sel], d)
+    rms64, _ = rel_err(a64[sel], d)
+    assert rms32 < 1e-3 and rms32 <= rms64 * 1.05, (rms32, rms64)
+
+
 def test_external_targets(native):
     x, m = workloads.plummer(50_000, seed=2)
     dt = native.DeviceTree.build(x, m, 16)
