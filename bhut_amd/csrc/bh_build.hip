@@ -489,18 +489,6 @@ __global__ __launch_bounds__(256) void k_leaf_up(
   }
 }
 
-// The node of slot `slot` of depth d among the deep-enough systems:
-// nthreads = 2^d x their number; lvl_sys lists them (NULL: one tree);
-// slot_off gives each system's first slot (NULL: one tree); slot_node maps a
-// slot to its node (-1: below a leaf).
-__device__ __forceinline__ int level_node(int d, int64_t t, const int32_t* __restrict__ lvl_sys,
-                                          const int64_t* __restrict__ slot_off,
-                                          const int32_t* __restrict__ slot_node) {
-  const int64_t sys = lvl_sys ? lvl_sys[t >> d] : 0;
-  const int64_t base = slot_off ? slot_off[sys] : 0;
-  return slot_node[base + (1ll << d) - 1 + (t & ((1ll << d) - 1))];
-}
-
 // Trees built here: one launch per depth, deepest first, so a node's
 // children are complete when it is combined (no atomics, no fences).
 template <int DIM, bool REFIT>

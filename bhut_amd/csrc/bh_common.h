@@ -65,6 +65,19 @@ struct __attribute__((aligned(64))) NodeD {
 };
 static_assert(sizeof(NodeD) == 64, "NodeD layout");
 
+// The node of thread t at depth d of the level-by-level upward passes:
+// nthreads = 2^d x the number of systems at least d deep; lvl_sys lists those
+// systems (NULL: one tree), slot_off gives each system's first slot (NULL:
+// one tree), slot_node maps a (depth, position) slot to its node (-1 below a
+// leaf).
+__device__ __forceinline__ int level_node(int d, int64_t t, const int32_t* __restrict__ lvl_sys,
+                                          const int64_t* __restrict__ slot_off,
+                                          const int32_t* __restrict__ slot_node) {
+  const int64_t sys = lvl_sys ? lvl_sys[t >> d] : 0;
+  const int64_t base = slot_off ? slot_off[sys] : 0;
+  return slot_node[base + (1ll << d) - 1 + (t & ((1ll << d) - 1))];
+}
+
 // ------------------------------------------------------------ context
 struct Ctx;
 

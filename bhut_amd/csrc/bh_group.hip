@@ -149,9 +149,7 @@ __global__ __launch_bounds__(256) void k_quad_level(int d, int64_t nthreads, con
                                                     const double* com, int dim, const double* mass, double* quad) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nthreads) return;
-  const int64_t sys = lvl_sys ? lvl_sys[t >> d] : 0;
-  const int64_t base = slot_off ? slot_off[sys] : 0;
-  const int v = slot_node[base + (1ll << d) - 1 + (t & ((1ll << d) - 1))];
+  const int v = level_node(d, t, lvl_sys, slot_off, slot_node);
   if (v < 0) return;
   if (first_child[v] < 0) quad_leaf(v, start, count, spart, com, dim, quad);
   else quad_node(v, first_child, com, dim, mass, quad);
