@@ -1,5 +1,6 @@
-// bh_build.hip -- tree topology, upward pass (geometry, mass, centre of mass),
-// traversal records, refit check.
+// bh_build.hip -- tree topology, upward pass (geometry, mass, centre of mass:
+// one launch per depth for trees built here, an atomic-flag climb for
+// imported ones), traversal records, refit check.
 //
 // Reference: _build_tree_iterative (bhut/tree/build.py:110-212),
 // _compute_node_geometry (build.py:215-242), _compute_mass_and_com

@@ -27,11 +27,13 @@
 //     FP32 on coordinates relative to the group centre: sources are stored
 //     relative to their leaf's centre of mass and shifted by the FP64-formed
 //     leaf-to-group offset, so precision follows the leaf size, not the
-//     coordinate size.  Each tile's partial sum is added to the per-target
-//     total with Kahan compensation.
+//     coordinate size.  A tile whose sources share one mass skips the
+//     per-pair mass product and scales its sum by the mass once.  Each tile's
+//     partial sum is added to the per-target total with Kahan compensation.
 //
 // Accepted cells carry their mass, centre of mass and second moments
-// (quadrupole, accumulated in FP64 by k_quad_up); the monopole-only variant
+// (quadrupole, accumulated in FP64 by k_quad_level one depth at a time, or
+// k_quad_up for imported trees); the monopole-only variant
 // is kept as an option.  FP64 stays in the tree build, the moments, the group
 // box and the centre/shift subtractions.
 #include "bh_common.h"
