@@ -127,6 +127,55 @@ def test_two_dimensional_and_zero_softening(native, masses):
         assert rms < 1e-5, (eps, rms)
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 65, 130])
+def test_small_and_partial_groups(native, n):
+    """N below, at and just above one group of 64 targets (a partly filled
+    last group, whose spare lanes hold copies of its first target): an FP32
+    direct sum at theta = 0, at theta = 0.5 no worse than the plain walk; a
+    lone particle has zero acceleration."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, 3))
+    m = rng.uniform(0.5, 1.5, n)
+    dt = native.DeviceTree.build(x, m, 8)
+    for theta in (0.0, 0.5):
+        a = dt.accel_f32(None, theta, 0.01, 1.0)
+        if n == 1:
+            assert np.all(a == 0)
+            continue
+        d = native.direct(x, m, softening=0.01)
+        rms, _ = rel_err(a, d)
+        rms64, _ = rel_err(dt.accel_self(theta, 0.01, 1.0), d)
+        assert rms <= max(rms64 * 1.05, 2e-6), (theta, rms, rms64)
+
+
+def test_coincident_particles_unsoftened(native):
+    """Two particles at one position with no softening: the pair has
+    r = 0 and adds nothing (the plain form's pure-Python leaf fallback,
+    bh.py:307-311, gives the same), in both the equal-mass and the per-pair
+    mass loops."""
+    x = np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 3.0], [2.0, 2.0, 3.0], [0.0, 0.5, 0.0]])
+    for m in (np.array([1.0, 2.0, 3.0, 4.0]), np.full(4, 0.25)):
+        dt = native.DeviceTree.build(x, m, 32)
+        a = dt.accel_f32(None, 0.5, 0.0, 1.0)
+        ref = dt.accel_self(0.5, 0.0, 1.0)
+        assert np.all(np.isfinite(a))
+        np.testing.assert_allclose(a, ref, rtol=1e-5, atol=1e-7)
+
+
+def test_far_external_targets(native):
+    """Targets around 1e6 times farther out than the sources, spread in all
+    directions (their bounding box encloses the whole system, so the exact
+    test decides): the root's moments serve them; FP32 relative error
+    against FP64 direct summation stays at the FP32 level."""
+    x, m = workloads.plummer(20_000, seed=14)
+    dt = native.DeviceTree.build(x, m, 32)
+    tg = np.random.default_rng(14).standard_normal((300, 3)) * 1e6
+    a = dt.accel_f32(tg, 0.5, 0.01, 1.0)
+    d = native.direct(x, m, tg, softening=0.01)
+    rms, mx = rel_err(a, d)
+    assert rms < 1e-5 and mx < 1e-4, (rms, mx)
+
+
 def test_refit_then_fp32(native):
     x, m = workloads.plummer(30_000, seed=6)
     dt = native.DeviceTree.build(x, m, 32)
