@@ -544,6 +544,7 @@ __device__ __forceinline__ void flush(float (&px)[TPL], float (&py)[TPL], float 
 
 
 constexpr int EV_WAVES = 4;  // groups (wavefronts) per workgroup
+constexpr int EV_XCDS = 8;   // item ranges of the evaluation: one per XCD of an MI355X
 
 // Work items.  A group's list is a sequence of tiles: its cell tiles (64
 // cells each) then its chunk tiles (64/CH chunks each).  Item j of a group
@@ -780,8 +781,9 @@ __device__ __forceinline__ void eval_item(
 
 // Persistent evaluation: as many workgroups as fit on the device at once;
 // each wavefront takes the next work item from a device counter until none
-// is left.  Items are numbered group by group (ioff[g] = first item of group
-// g, ioff[ng] = number of items).  *next must be zero at launch.
+// is left (one counter per item range, below).  Items are numbered group by
+// group (ioff[g] = first item of group g, ioff[ng] = number of items).
+// next[0 .. EV_XCDS) must be zero at launch.
 template <int TPL, int CH, bool ZEPS, bool SELF, bool KAHAN, bool QUAD>
 __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
     int64_t g0, int64_t ngroups, const int32_t* __restrict__ ioff, int64_t m, int gs,
@@ -798,27 +800,36 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const int lane = threadIdx.x % 64;
   const int64_t nitems = ioff[ngroups];
-  for (;;) {
-    uint32_t sl = 0;
-    if (lane == 0) sl = atomicAdd(next, 1u);
-    const int64_t item = (int64_t)__builtin_amdgcn_readfirstlane(sl);
-    if (item >= nitems) break;  // whole wavefront
-    // the item's group: last gi with ioff[gi] <= item
-    int64_t lo = 0, hi = ngroups - 1;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi + 1) / 2;
-      if (ioff[mid] <= item) lo = mid;
-      else hi = mid - 1;
+  // the items form EV_XCDS contiguous ranges of neighbouring groups, one per
+  // XCD (workgroups are dispatched to the XCDs in turn), so the groups whose
+  // lists share leaves run under one L2; a wavefront that finds its own
+  // range done takes items from the others'
+  const int x0 = (int)(blockIdx.x % EV_XCDS);
+  for (int r = 0; r < EV_XCDS; r++) {
+    const int x = (x0 + r) % EV_XCDS;
+    const int64_t lo_x = nitems * x / EV_XCDS, hi_x = nitems * (x + 1) / EV_XCDS;
+    for (;;) {
+      uint32_t sl = 0;
+      if (lane == 0) sl = atomicAdd(next + x, 1u);
+      const int64_t item = lo_x + (int64_t)__builtin_amdgcn_readfirstlane(sl);
+      if (item >= hi_x) break;  // whole wavefront
+      // the item's group: last gi with ioff[gi] <= item
+      int64_t lo = 0, hi = ngroups - 1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (ioff[mid] <= item) lo = mid;
+        else hi = mid - 1;
+      }
+      const int64_t g = g0 + lo;
+      const int64_t j = item - ioff[lo];
+      const int64_t tiles_g = (ccnt[g] + 63) / 64 + (kcnt[g] + CPT - 1) / CPT;
+      const int64_t ta = j * ITEM_TILES;
+      const int64_t tb = ta + ITEM_TILES < tiles_g ? ta + ITEM_TILES : tiles_g;
+      eval_item<TPL, CH, ZEPS, SELF, KAHAN, QUAD>(g, ta, tb, lane, tiles[w], m, gs, tpos, src32, pleaf, nodes,
+                                                  quad32, com, dim,
+                                                  boxes, coff, clist, koff, ccnt, kcnt, klist, eps2, gt0,
+                                                  part + item * (int64_t)(64 * TPL * 3));
     }
-    const int64_t g = g0 + lo;
-    const int64_t j = item - ioff[lo];
-    const int64_t tiles_g = (ccnt[g] + 63) / 64 + (kcnt[g] + CPT - 1) / CPT;
-    const int64_t ta = j * ITEM_TILES;
-    const int64_t tb = ta + ITEM_TILES < tiles_g ? ta + ITEM_TILES : tiles_g;
-    eval_item<TPL, CH, ZEPS, SELF, KAHAN, QUAD>(g, ta, tb, lane, tiles[w], m, gs, tpos, src32, pleaf, nodes,
-                                                quad32, com, dim,
-                                                boxes, coff, clist, koff, ccnt, kcnt, klist, eps2, gt0,
-                                                part + item * (int64_t)(64 * TPL * 3));
   }
 }
 
@@ -1306,13 +1317,13 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
   const bool zeps = !(eps2 > 0.f);
   const bool self = tgt == nullptr;
   int32_t* ioff = s.get<int32_t>(ng + 1);
-  uint32_t* next = s.get<uint32_t>(1);
+  uint32_t* next = s.get<uint32_t>(EV_XCDS);
   if (!ioff || !next) return BH_ENOMEM;
   // the evaluation and the reduction of groups [g0, g0 + nb): lists at the
   // offsets in offs, items[ioff], partial sums in part
   auto evaluate = [&](int64_t g0, int64_t nb, int64_t items, const CellEnt* clist, const ChunkEnt* klist,
                       double* part) -> int {
-    BH_TRY(fill_u32(c, next, 0u, 1));
+    BH_TRY(fill_u32(c, next, 0u, EV_XCDS));
     {
       ProfScope p(c, "k_group_eval_f32");
       const int64_t* coff = offs;
