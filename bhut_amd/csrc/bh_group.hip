@@ -801,13 +801,15 @@ __global__ __launch_bounds__(64 * EV_WAVES, 8) void k_group_eval(
   const int lane = threadIdx.x % 64;
   const int64_t nitems = ioff[ngroups];
   // the items form EV_XCDS contiguous ranges of neighbouring groups, one per
-  // XCD (workgroups are dispatched to the XCDs in turn), so the groups whose
-  // lists share leaves run under one L2; a wavefront that finds its own
-  // range done takes items from the others'
-  const int x0 = (int)(blockIdx.x % EV_XCDS);
-  for (int r = 0; r < EV_XCDS; r++) {
-    const int x = (x0 + r) % EV_XCDS;
-    const int64_t lo_x = nitems * x / EV_XCDS, hi_x = nitems * (x + 1) / EV_XCDS;
+  // XCD (workgroups are dispatched to the XCDs in turn), each with its own
+  // counter; a wavefront that finds its own range done takes items from the
+  // others'.  A small problem keeps one range (one counter attempt per
+  // wavefront at the end instead of EV_XCDS).
+  const int nx = nitems >= 2048 ? EV_XCDS : 1;
+  const int x0 = (int)(blockIdx.x % nx);
+  for (int r = 0; r < nx; r++) {
+    const int x = (x0 + r) % nx;
+    const int64_t lo_x = nitems * x / nx, hi_x = nitems * (x + 1) / nx;
     for (;;) {
       uint32_t sl = 0;
       if (lane == 0) sl = atomicAdd(next + x, 1u);
