@@ -112,6 +112,26 @@ def test_external_targets(native):
     assert rel_err(a, d)[0] <= rel_err(a64, d)[0] * 1.05
 
 
+@pytest.mark.parametrize("dim,theta", [(2, 0.5), (3, 0.3), (3, 1.0)])
+def test_external_targets_dims_and_thetas(native, dim, theta):
+    """External targets (no self exclusion, bh.py:30-80) in 2-D and 3-D and
+    at a tighter and a looser theta: an FP32 direct sum at theta = 0, and at
+    theta > 0 no less accurate than the plain walk against FP64 direct
+    summation (the group test opens only more than bhut's).  The 2-D
+    direct-sum bound is the one of the 2-D self test."""
+    rng = np.random.default_rng(40 + dim)
+    x = rng.standard_normal((20_000, dim))
+    m = rng.uniform(0.5, 1.5, 20_000)
+    tg = rng.uniform(-3, 3, (5_000, dim))
+    dt = native.DeviceTree.build(x, m, 16)
+    d = native.direct(x, m, tg, softening=0.01)
+    rms0, _ = rel_err(dt.accel_f32(tg, 0.0, 0.01, 1.0), d)
+    assert rms0 < (2e-6 if dim == 3 else 1e-5), rms0  # in-plane sums cancel more (2-D)
+    rms32, _ = rel_err(dt.accel_f32(tg, theta, 0.01, 1.0), d)
+    rms64, _ = rel_err(dt.accel_targets(tg, theta, 0.01, 1.0), d)
+    assert rms32 <= rms64 * 1.05, (rms32, rms64)
+
+
 @pytest.mark.parametrize("masses", ["random", "equal"])
 def test_two_dimensional_and_zero_softening(native, masses):
     """2-D, softened and unsoftened (the pair with r = 0 adds nothing), with
