@@ -52,6 +52,7 @@ _SIGS = {
     "bh_tree_shape": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
                                      ctypes.POINTER(i32), ctypes.POINTER(i32)]),
     "bh_tree_export": (ctypes.c_int, [vp] * 11),
+    "bh_tree_f32_stack": (ctypes.c_int, [vp, vp]),
     "bh_tree_should_refit": (ctypes.c_int, [vp, vp, i64, i32, ctypes.POINTER(i32)]),
     "bh_tree_should_refit_dev": (ctypes.c_int, [vp, vp, i64, i32, ctypes.POINTER(i32)]),
     "bh_tree_refit": (ctypes.c_int, [vp, vp, vp]),
@@ -193,6 +194,23 @@ def f64c(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _rows(a: np.ndarray, what: str, n: int | None, dim: int) -> np.ndarray:
+    """a as a contiguous float64 (rows, dim) array; ValueError otherwise (the
+    C side reads rows * dim values, so a wrong shape must stop here)."""
+    a = f64c(a)
+    if a.ndim != 2 or a.shape[1] != dim or (n is not None and a.shape[0] != n):
+        want = f"({n if n is not None else 'M'}, {dim})"
+        raise ValueError(f"{what} must have shape {want}, got {a.shape}")
+    return a
+
+
+def _vec(a: np.ndarray, what: str, n: int) -> np.ndarray:
+    a = f64c(a)
+    if a.shape != (n,):
+        raise ValueError(f"{what} must have shape ({n},), got {a.shape}")
+    return a
+
+
 class DeviceTree:
     """Owner of a bh_tree*; freed with the Python object."""
 
@@ -228,10 +246,18 @@ class DeviceTree:
     def from_arrays(cls, td, positions: np.ndarray, masses: np.ndarray,
                     context: Context | None = None) -> "DeviceTree":
         c = context or ctx()
-        pos, m = f64c(positions), f64c(masses)
         arrs = [f64c(td.center), f64c(td.half_size), f64c(td.mass), f64c(td.com)]
         ints = [np.ascontiguousarray(getattr(td, k), dtype=np.int64)
                 for k in ("first_child", "child_count", "start", "count", "perm")]
+        n, M, dim = len(ints[4]), len(arrs[2]), int(td.dim)
+        pos = _rows(positions, "positions", n, dim)
+        m = _vec(masses, "masses", n)
+        for name, a, shape in (("center", arrs[0], (M, dim)), ("half_size", arrs[1], (M,)),
+                               ("com", arrs[3], (M, dim)), ("first_child", ints[0], (M,)),
+                               ("child_count", ints[1], (M,)), ("start", ints[2], (M,)),
+                               ("count", ints[3], (M,))):
+            if a.shape != shape:
+                raise ValueError(f"{name} has shape {a.shape}, expected {shape}")
         h = vp()
         check(load().bh_tree_import(c.handle, len(ints[4]), len(arrs[2]), int(td.dim), int(td.leaf_size),
                                     *[ptr(a) for a in arrs], *[ptr(a) for a in ints], ptr(pos), ptr(m),
@@ -270,6 +296,12 @@ class DeviceTree:
         check(load().bh_tree_systems(self.handle, ctypes.byref(ns), ptr(po), ptr(no)))
         return po, no
 
+    def f32_stack(self) -> tuple[int, int]:
+        """LDS stack entries the FP32 walk grew to (self, external targets)."""
+        out = np.zeros(2, np.int32)
+        check(load().bh_tree_f32_stack(self.handle, ptr(out)))
+        return int(out[0]), int(out[1])
+
     def clone(self) -> "DeviceTree":
         h = vp()
         check(load().bh_tree_clone(self.handle, ctypes.byref(h)))
@@ -290,13 +322,15 @@ class DeviceTree:
 
     def should_refit(self, positions: np.ndarray) -> bool:
         pos = f64c(positions)
+        if pos.ndim != 2:
+            raise ValueError(f"positions must be 2D array, got shape {pos.shape}")
         ok = i32()
         check(load().bh_tree_should_refit(self.handle, ptr(pos), pos.shape[0], pos.shape[1], ctypes.byref(ok)))
         return bool(ok.value)
 
     def refit(self, positions: np.ndarray, masses: np.ndarray | None) -> None:
-        pos = f64c(positions)
-        m = f64c(masses) if masses is not None else None
+        pos = _rows(positions, "positions", self.n, self.dim)
+        m = _vec(masses, "masses", self.n) if masses is not None else None
         check(load().bh_tree_refit(self.handle, ptr(pos), ptr(m)))
 
     def accel_self(self, theta: float, softening: float, G: float) -> np.ndarray:
@@ -332,7 +366,7 @@ class DeviceTree:
         check(load().bh_tree_refit_dev(self.handle, vp(pos_ptr), vp(mass_ptr) if mass_ptr else None))
 
     def accel_targets(self, targets: np.ndarray, theta: float, softening: float, G: float) -> np.ndarray:
-        tgt = f64c(targets)
+        tgt = _rows(targets, "targets", None, self.dim)
         out = np.empty_like(tgt)
         if len(tgt):
             check(load().bh_accel_targets(self.handle, ptr(tgt), len(tgt), float(theta), float(softening),
@@ -348,7 +382,7 @@ class DeviceTree:
             check(load().bh_accel_f32_self(self.handle, float(theta), float(softening), float(G), op, ptr(out),
                                            ptr(st)))
         else:
-            tgt = f64c(targets)
+            tgt = _rows(targets, "targets", None, self.dim)
             out = np.empty_like(tgt)
             if len(tgt):
                 check(load().bh_accel_f32_targets(self.handle, ptr(tgt), len(tgt), float(theta),
@@ -357,7 +391,7 @@ class DeviceTree:
 
     def potential(self, targets, theta: float, softening: float, G: float) -> np.ndarray:
         """Softened potential at the sources (targets None) or at targets."""
-        tgt = None if targets is None else f64c(targets)
+        tgt = None if targets is None else _rows(targets, "targets", None, self.dim)
         m = self.n if tgt is None else len(tgt)
         out = np.empty(m)
         if m:
@@ -371,8 +405,8 @@ class DeviceTree:
 
     def interaction_lists(self, targets, theta: float):
         """(ptr, ids) of per-target interaction lists in pop order."""
-        m = self.n if targets is None else len(targets)
-        tgt = None if targets is None else f64c(targets)
+        tgt = None if targets is None else _rows(targets, "targets", None, self.dim)
+        m = self.n if tgt is None else len(tgt)
         p = np.zeros(m + 1, np.int64)
         total = i64()
         check(load().bh_interaction_lists(self.handle, ptr(tgt), m, float(theta), ptr(p), None, 0,
@@ -383,8 +417,8 @@ class DeviceTree:
         return p, ids[:total.value].astype(np.int64)
 
     def counters(self, targets, theta: float) -> np.ndarray:
-        m = self.n if targets is None else len(targets)
-        tgt = None if targets is None else f64c(targets)
+        tgt = None if targets is None else _rows(targets, "targets", None, self.dim)
+        m = self.n if tgt is None else len(tgt)
         out = np.zeros((m, 5), np.int64)
         check(load().bh_counters(self.handle, ptr(tgt), m, float(theta), ptr(out)))
         return out

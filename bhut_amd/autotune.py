@@ -2,10 +2,12 @@
 
 Times the group walk and evaluation kernels (HIP events recorded by the
 library on its stream) for each candidate configuration on the caller's own
-tree and input, and returns the fastest.  Candidates whose LDS stack is too
-small for the tree (the walk reports an overflow) are skipped.  Leaf size
-changes the tree itself, so `tune_leaf_f32` rebuilds per candidate and also
-times the build.
+tree and input, and returns the fastest.  Each stack candidate runs on a
+fresh clone of the tree: the library grows a walk's LDS stack when it
+overflows and later walks on the same tree start at the grown size, so the
+table reports the stack each candidate actually ran with
+(``stack_used``).  Leaf size changes the tree itself, so `tune_leaf_f32`
+rebuilds per candidate and also times the build.
 """
 
 from __future__ import annotations
@@ -16,7 +18,7 @@ import numpy as np
 
 from . import _native
 
-_F32_KERNELS = ("k_group_walk_count", "k_group_walk_fill", "k_group_order", "k_group_eval_f32", "f32_prepare")
+_F32_KERNELS = ("k_group_walk_count", "k_group_walk_fill", "k_group_eval_f32", "k_item_reduce", "f32_prepare")
 
 
 def _timed(ctx, fn, reps: int) -> float:
@@ -40,10 +42,11 @@ def tune_f32(tree: "_native.DeviceTree", theta: float, softening: float, G: floa
     for gs in group_sizes:
         for sc in stack_caps:
             opts = _native.F32Opts(int(gs), int(kahan), int(sc), int(multipole))
+            cand = tree.clone()  # starts with no grown stack of its own
             if out_ptr is not None and targets is None:
-                fn = lambda: tree.accel_f32_self_dev(out_ptr, theta, softening, G, opts)
+                fn = lambda: cand.accel_f32_self_dev(out_ptr, theta, softening, G, opts)
             else:
-                fn = lambda: tree.accel_f32(targets, theta, softening, G, opts)
+                fn = lambda: cand.accel_f32(targets, theta, softening, G, opts)
             try:
                 ms = _timed(ctx, fn, reps)
             except RuntimeError as e:
@@ -51,12 +54,14 @@ def tune_f32(tree: "_native.DeviceTree", theta: float, softening: float, G: floa
                     raise
                 table.append({"group_size": gs, "stack_cap": sc, "ms": None, "note": "stack overflow"})
                 continue
-            table.append({"group_size": gs, "stack_cap": sc, "ms": ms})
+            grown = cand.f32_stack()[0 if targets is None else 1]
+            table.append({"group_size": gs, "stack_cap": sc, "stack_used": max(int(sc), grown), "ms": ms})
+            del cand
     ok = [r for r in table if r["ms"] is not None]
     if not ok:
         raise RuntimeError("no FP32 configuration fits; raise the stack candidates")
     best = min(ok, key=lambda r: r["ms"])
-    return {"group_size": best["group_size"], "stack_cap": best["stack_cap"]}, table
+    return {"group_size": best["group_size"], "stack_cap": best["stack_used"]}, table
 
 
 def tune_leaf_f32(positions, masses, theta: float, softening: float, G: float = 1.0, *,

@@ -531,6 +531,13 @@ int bh_tree_shape(const bh_tree* t, int64_t* n, int64_t* num_nodes, int32_t* dim
   return BH_OK;
 }
 
+int bh_tree_f32_stack(const bh_tree* t, int32_t* out) {
+  BH_REQ(t && out, "tree and out must not be NULL");
+  out[0] = t->stack_hint[0];
+  out[1] = t->stack_hint[1];
+  return BH_OK;
+}
+
 int bh_tree_export(const bh_tree* t, double* center, double* half_size, double* mass, double* com,
                    int64_t* first_child, int64_t* child_count, int64_t* start, int64_t* count,
                    uint8_t* is_leaf, int64_t* perm) {

@@ -1139,6 +1139,9 @@ static void launch_eval(bh_ctx* c, int64_t g0, int64_t ng, const int32_t* ioff, 
 int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, double eps, double G,
              const bh_f32_opts* o, double* out, int64_t* stats, const F32Plan* plan) {
   if (m == 0) return BH_OK;
+  // the group walk pushes both children of an opened node (first_child and
+  // first_child + 1); an imported tree with other child counts is refused
+  BH_REQ(t->binary, "the FP32 form needs a binary tree (every internal node with child_count 2)");
   const int gs = o && o->group_size ? o->group_size : 64;
   const bool kahan = o ? o->kahan != 0 : true;
   // LDS stack entries per wavefront, a multiple of 16 so what follows the
@@ -1557,6 +1560,18 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
       BH_LAUNCH_CHECK();
     }
     BH_TRY(evaluate(b.g0, nb, b.items, clist, klist, part));
+  }
+  {
+    // the fill walks repeat the count pass's walks with the same stack, so
+    // they cannot overflow; were a bit set anyway, the evaluation and the
+    // reduction skipped and out holds no result
+    uint32_t herr = 0;
+    BH_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(hipStreamSynchronize(c->stream));
+    if (herr) {
+      set_error("FP32 group walk reported status %u while filling its lists", herr);
+      return BH_EOVERFLOW;
+    }
   }
   if (stats) {
     stats[0] = cell_int;

@@ -64,6 +64,8 @@ def barnes_hut_accelerations_targets(tree: TreeData, tree_positions, tree_masses
     validate_kernel_inputs(tree_positions, tree_masses, softening)
     eps = _scalar_softening(softening)
     tgt = np.asarray(target_positions, dtype=np.float64)
+    if tgt.ndim != 2 or tgt.shape[1] != tree_positions.shape[1]:
+        raise ValueError(f"target_positions must have shape (M, {tree_positions.shape[1]}), got {tgt.shape}")
     dt = device_tree_for(tree, tree_positions, tree_masses)
     if precision == "fp64":
         return dt.accel_targets(tgt, theta, eps, G)
@@ -79,27 +81,62 @@ def interaction_lists(tree: TreeData, positions, masses, theta: float = 0.5, tar
     return dt.interaction_lists(None if targets is None else np.asarray(targets, np.float64), theta)
 
 
-def compute_tree_statistics(tree: TreeData, positions, theta: float = 0.5, masses=None) -> Dict[str, Any]:
-    """Traversal statistics with exact device counters.  Keys of the
-    reference (bh.py:604-646) plus exact extras; 'avg_particles_computed'
-    counts the pairs actually summed (the reference counts count-1 per leaf
-    whether or not the target lies in it, bh.py:678-681)."""
+def _counting_tree(tree: TreeData):
+    """A device tree whose node records are `tree`'s.  The counters read only
+    node records (mass, centre of mass, size, children, particle ranges), so a
+    host-only TreeData is imported with zero source arrays."""
+    dt = tree.device_tree
+    if dt is not None:
+        return dt
+    from .. import _native
+    n = tree.num_particles
+    return _native.DeviceTree.from_arrays(tree, np.zeros((n, tree.dim)), np.zeros(n))
+
+
+def traversal_counters(tree: TreeData, positions, theta: float = 0.5, *, self_mode: bool = False) -> np.ndarray:
+    """Exact per-target counts of the walk, (m, 5) int64: nodes popped,
+    opening tests, monopoles accepted, leaves visited, particle pairs summed.
+    With self_mode, positions must be the tree's own particles in original
+    order and target i skips source i (bh.py:253-255); otherwise nothing is
+    skipped (bh.py:196-221)."""
     positions = np.asarray(positions, dtype=np.float64)
-    if masses is None:
-        dt = tree.device_tree
-        if dt is None:
-            raise ValueError("masses are needed for a tree without a device copy")
-    else:
-        dt = device_tree_for(tree, positions, np.asarray(masses, dtype=np.float64))
-    c = dt.counters(None, theta) if len(positions) == dt.n else dt.counters(positions, theta)
+    if not self_mode:
+        return _counting_tree(tree).counters(positions, theta)
+    dt = tree.device_tree
+    if dt is None or getattr(dt, "src_pos", None) is not positions:
+        from .. import _native
+        dt = _native.DeviceTree.from_arrays(tree, positions, np.zeros(len(positions)))
+    return dt.counters(None, theta)
+
+
+def compute_tree_statistics(tree: TreeData, positions, theta: float = 0.5) -> Dict[str, Any]:
+    """compute_tree_statistics (bh.py:604-646) with the reference's values.
+
+    As in _count_operations_single_target (bh.py:648-700), the walk runs at
+    each passed position (the index is not used): 'avg_nodes_visited' counts
+    every pop, zero-mass nodes included; 'avg_particles_computed' adds
+    count - 1 for every visited leaf (bh.py:678-681) whether or not the
+    position is one of the leaf's particles; 'total_operations' is particles
+    plus monopoles.  The exact counts of the same walks are added under keys
+    of their own: 'avg_opening_tests', 'avg_leaves_visited',
+    'total_pairs_exact' (the particles a target at that position sums, no
+    self exclusion) and 'per_target' ((m, 5) int64, see traversal_counters).
+    """
+    positions = np.asarray(positions, dtype=np.float64)
     n = len(positions)
-    tot = c.sum(axis=0)
+    if n == 0:
+        raise ZeroDivisionError("division by zero")  # total / n_particles with no positions (bh.py:641)
+    c = _counting_tree(tree).counters(positions, theta)
+    tot = [int(v) for v in c.sum(axis=0)]
+    particles = tot[4] - tot[3]  # sum over visited leaves of (count - 1)
+    monopoles = tot[2]
     return {
         "avg_nodes_visited": tot[0] / n,
-        "avg_particles_computed": tot[4] / n,
-        "avg_monopoles_used": tot[2] / n,
-        "total_operations": int(tot[4] + tot[2]),
+        "avg_particles_computed": particles / n,
+        "avg_monopoles_used": monopoles / n,
+        "total_operations": particles + monopoles,
         "avg_opening_tests": tot[1] / n,
         "avg_leaves_visited": tot[3] / n,
+        "total_pairs_exact": tot[4],
         "per_target": c,
     }

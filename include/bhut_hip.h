@@ -101,6 +101,11 @@ int bh_tree_systems(const bh_tree *t, int64_t *nsys, int64_t *particle_offsets, 
 void bh_tree_free(bh_tree *t);
 int bh_tree_shape(const bh_tree *t, int64_t *n, int64_t *num_nodes, int32_t *dim,
                   int32_t *leaf_size, int32_t *depth);
+/* LDS stack entries the FP32 group walk on this tree has grown to, for self
+ * (out[0]) and external (out[1]) targets; 0 where it never had to grow.
+ * Later walks on the tree start there.  No reference counterpart: the
+ * autotuner reads it to report the stack a candidate actually ran with. */
+int bh_tree_f32_stack(const bh_tree *t, int32_t *out);
 /* TreeData arrays (node.py:59-77) copied to host buffers; any may be NULL. */
 int bh_tree_export(const bh_tree *t, double *center, double *half_size, double *mass, double *com,
                    int64_t *first_child, int64_t *child_count, int64_t *start, int64_t *count,

@@ -278,3 +278,17 @@ def direct_potential(src_pos, src_mass, targets=None, *, softening=0.0, G=1.0, n
     lib().orc_direct_potential(_p(src_pos), _p(src_mass), len(src_pos), _p(tgt), len(tgt), tgt.shape[1],
                                int(self_mode), float(softening), float(G), int(nthreads), _p(out))
     return out
+
+
+def statistics(tree: OracleTree, positions, theta: float) -> dict:
+    """compute_tree_statistics (bhut/traverse/bh.py:604-646) from the exact
+    counters of external-mode walks at `positions`: the reference's
+    _count_operations_single_target (bh.py:648-700) runs the opening test at
+    positions[i] and never uses the index, counts every pop, and adds
+    count - 1 for each visited leaf (bh.py:678-681), i.e. pairs - leaves."""
+    c = counters(tree, positions, self_mode=False, theta=theta)
+    n = len(c)
+    tot = [int(v) for v in c.sum(axis=0)]
+    particles = tot[4] - tot[3]
+    return {"avg_nodes_visited": tot[0] / n, "avg_particles_computed": particles / n,
+            "avg_monopoles_used": tot[2] / n, "total_operations": particles + tot[2]}

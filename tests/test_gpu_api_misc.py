@@ -26,20 +26,35 @@ def bits(a):
 
 
 def test_statistics_and_refit_rule(native):
-    """compute_tree_statistics (bh.py:604-700) from exact counters equal to
-    the oracle's; should_refit_vs_rebuild (refit.py:195-251) agrees with the
-    oracle on a small drift, a far particle and a changed N."""
-    from bhut_amd.traverse.bh import compute_tree_statistics
+    """compute_tree_statistics (bh.py:604-700) equal to the oracle's
+    restatement of the reference's values, at the tree's positions, at
+    displaced positions of the same count (the walk runs at the passed
+    positions) and on a host-only TreeData; traversal_counters in self mode
+    equal to the oracle's exact counts; should_refit_vs_rebuild
+    (refit.py:195-251) agrees with the oracle on a small drift, a far
+    particle and a changed N."""
+    from bhut_amd.traverse.bh import compute_tree_statistics, traversal_counters
     from bhut_amd.tree.build import build_tree
+    from bhut_amd.tree.node import TreeData
     from bhut_amd.tree.refit import should_refit_vs_rebuild
     x, m = workloads.plummer(5000, seed=21)
     tree = build_tree(x, m, dim=3, leaf_size=16)
     ot = O.build(x, m, 16)
-    st = compute_tree_statistics(tree, x, 0.5)
-    c = O.counters(ot, x, self_mode=True, theta=0.5)
-    np.testing.assert_array_equal(st["per_target"], c)
-    assert st["total_operations"] == int(c[:, 4].sum() + c[:, 2].sum())
-    assert st["avg_nodes_visited"] == c[:, 0].sum() / len(x)
+    host = TreeData(**tree.arrays(), dim=3, leaf_size=16)
+    xd = x + 0.05 * np.random.default_rng(21).standard_normal(x.shape)
+    for tr in (tree, host):
+        for p in (x, xd, x[:700]):
+            st = compute_tree_statistics(tr, p, 0.5)
+            exp = O.statistics(ot, p, 0.5)
+            for k, v in exp.items():
+                assert st[k] == v, k
+            np.testing.assert_array_equal(st["per_target"], O.counters(ot, p, self_mode=False, theta=0.5))
+    np.testing.assert_array_equal(traversal_counters(tree, x, 0.5, self_mode=True),
+                                  O.counters(ot, x, self_mode=True, theta=0.5))
+    np.testing.assert_array_equal(traversal_counters(host, x, 0.5, self_mode=True),
+                                  O.counters(ot, x, self_mode=True, theta=0.5))
+    with pytest.raises(ValueError):
+        compute_tree_statistics(tree, x[:, :2], 0.5)
     x2 = x + 1e-4
     assert should_refit_vs_rebuild(tree, x2) and O.should_refit(ot, x2)
     x3 = x.copy()
@@ -122,3 +137,73 @@ def test_autotuner_and_forest_bookkeeping(native):
     native.ctx().trim()
     a = native.DeviceTree.build(x, m, 32).accel_self(0.5, 0.01, 1.0)
     assert np.all(np.isfinite(a))
+
+
+def test_misshaped_inputs_raise_before_any_device_call(native):
+    """Targets, masses and positions whose shapes do not match the tree
+    raise ValueError on the host (the C side reads rows * dim values, so a
+    wrong shape must never reach it), as the reference's broadcasting fails
+    with ValueError."""
+    import bhut_amd
+    from bhut_amd.batched import Forest
+    from bhut_amd.traverse.bh import barnes_hut_accelerations_targets
+    from bhut_amd.tree.build import build_tree
+    from bhut_amd.tree.node import TreeData
+    x, m = workloads.plummer(3000, seed=31)
+    tree = build_tree(x, m, dim=3, leaf_size=16)
+    dt = tree.device_tree
+    bad_targets = (np.zeros((10, 2)), np.zeros(30), np.zeros((10, 4)), np.zeros((2, 5, 3)))
+    for tg in bad_targets:
+        for call in (lambda: dt.accel_targets(tg, 0.5, 0.01, 1.0), lambda: dt.accel_f32(tg, 0.5, 0.01, 1.0),
+                     lambda: dt.potential(tg, 0.5, 0.01, 1.0), lambda: dt.counters(tg, 0.5),
+                     lambda: dt.interaction_lists(tg, 0.5),
+                     lambda: barnes_hut_accelerations_targets(tree, x, m, tg, 0.01, 0.5)):
+            with pytest.raises(ValueError):
+                call()
+    with pytest.raises(ValueError):
+        dt.refit(x[:100], m[:100])
+    with pytest.raises(ValueError):
+        dt.refit(x, m[:100])
+    with pytest.raises(ValueError):
+        dt.refit(x[:, :2], m)
+    host = TreeData(**tree.arrays(), dim=3, leaf_size=16)
+    with pytest.raises(ValueError):
+        native.DeviceTree.from_arrays(host, x[:100], m[:100])
+    with pytest.raises(ValueError):
+        native.DeviceTree.from_arrays(host, x, m[:10])
+    f = Forest(x, m, [1000, 2000], leaf_size=16)
+    with pytest.raises(ValueError):
+        f.refit(x, m[:5])
+    t = bhut_amd.Tree(x, m, leaf_size=16).build()
+    with pytest.raises(ValueError):
+        t.accelerations(np.zeros((5, 2)))
+    # the tree still works after the refused calls
+    np.testing.assert_array_equal(dt.accel_targets(x[:7], 0.5, 0.01, 1.0), dt.accel_targets(x[:7], 0.5, 0.01, 1.0))
+
+
+def test_three_child_tree_plain_walk_and_fp32_refusal(native):
+    """An imported TreeData whose root has three children (bhut's walk pushes
+    child_count children, bh.py:529-535): the plain walk visits all three and
+    matches the oracle at theta 0 and 1; the FP32 form, whose group walk
+    pushes exactly two children, refuses the tree with ValueError."""
+    from bhut_amd.traverse.bh import barnes_hut_accelerations
+    from bhut_amd.tree.node import TreeData
+    x = np.array([[0.0, 0.0, 0.0], [1.0, 0.5, 0.0], [3.0, -1.0, 2.0]])
+    m = np.array([1.0, 2.0, 0.5])
+    lo, hi = x.min(axis=0), x.max(axis=0)
+    center = np.vstack([(lo + hi) / 2, x])
+    half = np.array([np.max(hi - lo) / 2, 0.0, 0.0, 0.0])
+    mass = np.array([m.sum(), *m])
+    com = np.vstack([(m[:, None] * x).sum(axis=0) / m.sum(), x])
+    td = TreeData(center=center, half_size=half, mass=mass, com=com, first_child=np.array([1, -1, -1, -1]),
+                  child_count=np.array([3, 0, 0, 0]), start=np.array([0, 0, 1, 2]), count=np.array([3, 1, 1, 1]),
+                  is_leaf=np.array([False, True, True, True]), perm=np.arange(3), dim=3, leaf_size=1)
+    ot = O.OracleTree(center=center, half_size=half, mass=mass, com=com, first_child=np.array([1, -1, -1, -1]),
+                      child_count=np.array([3, 0, 0, 0]), start=np.array([0, 0, 1, 2]),
+                      count=np.array([3, 1, 1, 1]), is_leaf=np.array([False, True, True, True]),
+                      perm=np.arange(3), dim=3, leaf_size=1)
+    for theta in (0.0, 1.0, 10.0):
+        got = barnes_hut_accelerations(td, x, m, 0.01, theta)
+        np.testing.assert_allclose(got, O.accelerations(ot, x, m, softening=0.01, theta=theta), rtol=1e-12)
+    with pytest.raises(ValueError, match="binary"):
+        barnes_hut_accelerations(td, x, m, 0.01, 0.5, precision="fp32")

@@ -99,6 +99,88 @@ def test_golden_lists_and_accelerations(native, golden, eps):
                 np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], gi[gp[i]:gp[i + 1]])
 
 
+REF_STAT_KEYS = ("avg_nodes_visited", "avg_particles_computed", "avg_monopoles_used", "total_operations")
+
+
+def ref_files():
+    d = os.path.join(os.path.dirname(__file__), "golden")
+    return sorted(os.path.join(d, f) for f in os.listdir(d)
+                  if f.startswith("ref_") and f.endswith(".npz") and f != "ref_normal10k.npz")
+
+
+@pytest.mark.parametrize("path", ref_files(), ids=lambda p: os.path.basename(p)[4:-4])
+def test_reference_walk_fixtures(native, path):
+    """The HIP plain form against outputs the reference's own walk wrote
+    (tests/golden/make_golden_ref.py): TreeData bit for bit, per-target
+    lists in pop order, pops / opening tests / monopoles as the reference
+    counts them, compute_tree_statistics exactly (also at displaced
+    positions of the same count), accelerations within 1e-12 of both leaf
+    formulas."""
+    from bhut_amd.traverse.bh import compute_tree_statistics
+    from bhut_amd.tree.build import build_tree
+    g = np.load(path, allow_pickle=False)
+    pos, mass, ext = g["pos"], g["mass"], g["ext"]
+    n, dim = pos.shape
+    for leaf in g["leaves"]:
+        tree = build_tree(pos, mass, dim=dim, leaf_size=int(leaf))
+        dt = tree.device_tree
+        e = dt.export()
+        for f in TREE_FIELDS:
+            bits_equal(e[f], g[f"L{leaf}_{f}"], f"L{leaf} {f}")
+        for theta in g["thetas"]:
+            tag = f"L{leaf}_t{theta}"
+            ops = g[f"{tag}_ops"]
+            nt = len(ops)
+            c = dt.counters(None, theta)[:nt]
+            np.testing.assert_array_equal(c[:, 0], ops[:, 0], err_msg=tag)
+            np.testing.assert_array_equal(c[:, 2], ops[:, 2], err_msg=tag)
+            np.testing.assert_array_equal(c[:, 1], g[f"{tag}_e0.0_tests"], err_msg=tag)
+            ce = dt.counters(ext, theta)
+            np.testing.assert_array_equal(ce[:, [0, 2]], g[f"{tag}_ext_ops"][:, [0, 2]], err_msg=tag)
+            np.testing.assert_array_equal(ce[:, 4] - ce[:, 3], g[f"{tag}_ext_ops"][:, 1], err_msg=tag)
+            for key, p in (("stats", pos), ("stats_pos2", g["pos2"])):
+                st = compute_tree_statistics(tree, p, theta)
+                np.testing.assert_array_equal(np.array([float(st[k]) for k in REF_STAT_KEYS]),
+                                              g[f"{tag}_{key}"], err_msg=f"{tag} {key}")
+            ptr, ids = dt.interaction_lists(None, theta)
+            gp, gi = g[f"{tag}_e0.0_list_ptr"], g[f"{tag}_e0.0_list_ids"]
+            for i in range(nt):
+                np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], gi[gp[i]:gp[i + 1]], err_msg=f"{tag} t{i}")
+            ptr, ids = dt.interaction_lists(ext, theta)
+            gp, gi = g[f"{tag}_e0.0_ext_list_ptr"], g[f"{tag}_e0.0_ext_list_ids"]
+            for i in range(len(ext)):
+                np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], gi[gp[i]:gp[i + 1]], err_msg=f"{tag} e{i}")
+            for eps in (0.0, 0.01):
+                tag = f"L{leaf}_t{theta}_e{eps}"
+                acc = dt.accel_self(theta, eps, 1.0)[:nt]
+                acc_close(acc, g[f"{tag}_acc_numba"])
+                acc_close(acc, g[f"{tag}_acc_python"])
+                acc_e = dt.accel_targets(ext, theta, eps, 1.0)
+                acc_close(acc_e, g[f"{tag}_ext_acc_numba"])
+                acc_close(acc_e, g[f"{tag}_ext_acc_python"])
+
+
+def test_reference_normal10k_public_api(native):
+    """bhut_amd.accelerations on the CPU-runnable workload (10 000
+    standard-normal particles, seed 0, unit masses, theta 0.5, softening
+    0.01, leaf 32) against bhut.accelerations' own output for every
+    particle, within 1e-12; sampled lists and the statistics exactly."""
+    import bhut_amd
+    from bhut_amd.traverse.bh import compute_tree_statistics
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_normal10k.npz"), allow_pickle=False)
+    x, m = g["pos"], g["mass"]
+    acc_close(bhut_amd.accelerations(x, m, theta=0.5, softening=0.01, G=1.0, leaf_size=32), g["acc"])
+    t = bhut_amd.Tree(x, m, leaf_size=32).build()
+    sel = g["sel"]
+    acc_close(t.accelerations(theta=0.5, softening=0.01)[sel], g["sel_acc_numba"])
+    ptr, ids = t._tree.device_tree.interaction_lists(None, 0.5)
+    gp, gi = g["sel_list_ptr"], g["sel_list_ids"]
+    for k, i in enumerate(sel):
+        np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], gi[gp[k]:gp[k + 1]])
+    st = compute_tree_statistics(t._tree, x, 0.5)
+    np.testing.assert_array_equal(np.array([float(st[k]) for k in REF_STAT_KEYS]), g["stats"])
+
+
 def plummer(n, seed, a=1.0, rmax=100.0):
     rng = np.random.default_rng(seed)
     r = np.empty(n)
