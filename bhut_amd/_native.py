@@ -61,6 +61,7 @@ _SIGS = {
     "bh_accel_self_dev": (ctypes.c_int, [vp, dbl, dbl, dbl, vp]),
     "bh_accel_targets": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp]),
     "bh_accel_targets_dev": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp]),
+    "bh_accel_targets_dev_noalloc": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp]),
     "bh_accel_f32_self": (ctypes.c_int, [vp, dbl, dbl, dbl, vp, vp, vp]),
     "bh_accel_f32_self_dev": (ctypes.c_int, [vp, dbl, dbl, dbl, vp, vp, vp]),
     "bh_accel_f32_targets": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp, vp, vp]),
@@ -346,6 +347,13 @@ class DeviceTree:
                           G: float) -> None:
         check(load().bh_accel_targets_dev(self.handle, vp(tgt_ptr), int(m), float(theta), float(softening),
                                           float(G), vp(out_ptr)))
+
+    def accel_targets_dev_noalloc(self, tgt_ptr: int, m: int, out_ptr: int, theta: float, softening: float,
+                                  G: float) -> None:
+        """accel_targets_dev without device allocation (at most 2**20 targets
+        per call; calls on one device must not overlap)."""
+        check(load().bh_accel_targets_dev_noalloc(self.handle, vp(tgt_ptr), int(m), float(theta),
+                                                  float(softening), float(G), vp(out_ptr)))
 
     def accel_f32_self_dev(self, out_ptr: int, theta: float, softening: float, G: float,
                            opts: F32Opts | None = None) -> np.ndarray:

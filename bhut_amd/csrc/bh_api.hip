@@ -310,6 +310,14 @@ int bh_ctx_create(int device, bh_ctx** out) {
     c->num_cus = prop.multiProcessorCount;
     c->mem_total = prop.totalGlobalMem;
   }
+  // the no-allocation entry point's workspace on this device (looked up
+  // once per device, with the device current)
+  const int rc = noalloc_setup(device);
+  if (rc != BH_OK) {
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return BH_OK;
 }
@@ -850,6 +858,15 @@ int bh_accel_targets_dev(bh_tree* t, const double* d_tgt, int64_t m, double thet
   BH_REQ(t->nsys == 1, "external targets need a single tree, not a forest");
   DeviceGuard g(t->ctx->device);
   return walk_mode(t->ctx, t, WALK_ACC, d_tgt, m, theta, eps, G, d_out, nullptr, nullptr, nullptr);
+}
+
+int bh_accel_targets_dev_noalloc(bh_tree* t, const double* d_tgt, int64_t m, double theta, double eps,
+                                 double G, double* d_out) {
+  BH_REQ(t && (m == 0 || (d_tgt && d_out)), "NULL argument");
+  BH_REQ(m >= 0, "bad target count");
+  BH_REQ(t->nsys == 1, "external targets need a single tree, not a forest");
+  DeviceGuard g(t->ctx->device);
+  return walk_targets_noalloc(t->ctx, t, d_tgt, m, theta, eps, G, d_out);
 }
 
 int bh_accel_targets(bh_tree* t, const double* tgt, int64_t m, double theta, double eps, double G,

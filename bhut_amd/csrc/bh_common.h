@@ -244,6 +244,13 @@ int morton_iota(bh_ctx* c, const double* pos, int64_t n, int dim, const double* 
                 uint64_t* keys, uint32_t* vals);
 int radix_sort(bh_ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int bits, uint64_t** keys_out,
                uint32_t** vals_out, uint64_t* keys_alt, uint32_t* vals_alt);
+// the same sort in a caller's workspace: counts holds 256 * radix_tiles(n)
+// entries, totals 256 (both written before they are read)
+constexpr int RADIX_TILE = 4096;  // keys per tile of the digit histograms
+inline int64_t radix_tiles(int64_t n) { return (n + RADIX_TILE - 1) / RADIX_TILE; }
+int radix_sort_ws(bh_ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int bits, uint64_t** keys_out,
+                  uint32_t** vals_out, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* counts,
+                  uint32_t* totals);
 int iota(bh_ctx* c, uint32_t* v, int64_t n);
 int gather(bh_ctx* c, const double* pos, const double* mass, const uint32_t* perm, int64_t n,
            int dim, double4* spart);
@@ -259,6 +266,10 @@ int fill_u32(bh_ctx* c, uint32_t* p, uint32_t v, int64_t n);
 // bh_walk.hip
 enum { WALK_ACC = 0, WALK_LCOUNT = 1, WALK_LFILL = 2, WALK_STATS = 3, WALK_POT = 4 };
 int order_targets(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, uint32_t* torder);
+// device scratch of the no-allocation entry point, looked up once per device
+int noalloc_setup(int device);
+int walk_targets_noalloc(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, double theta, double eps,
+                         double G, double* out);
 int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t m, double theta,
               double eps, double G, double* out, const int64_t* lptr, int32_t* lids, int64_t* stats);
 // bh_group.hip

@@ -292,6 +292,7 @@ int morton_iota(bh_ctx* c, const double* pos, int64_t n, int dim, const double* 
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per tile
+static_assert(RS_TILE == RADIX_TILE, "radix tile size");
 constexpr int RS_WAVES = RS_THREADS / WAVE;
 
 // Per-tile digit counts, stored digit-major: counts[d * ntiles + tile], so
@@ -471,17 +472,14 @@ int iota(bh_ctx* c, uint32_t* v, int64_t n) {
 // Sorts (keys, vals) by the low `bits` bits of the keys, stably.  Uses the
 // alt buffers as ping-pong partners; *keys_out/*vals_out receive whichever
 // buffer holds the result.
-int radix_sort(bh_ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int bits, uint64_t** keys_out,
-               uint32_t** vals_out, uint64_t* keys_alt, uint32_t* vals_alt) {
+int radix_sort_ws(bh_ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int bits, uint64_t** keys_out,
+                  uint32_t** vals_out, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* counts,
+                  uint32_t* totals) {
   *keys_out = keys;
   *vals_out = vals;
   if (n <= 1) return BH_OK;
   if (n > 0xffffffffll) return BH_EINVAL;
-  const int ntiles = (int)cdiv(n, RS_TILE);
-  Scratch s(c);
-  uint32_t* counts = s.get<uint32_t>((size_t)256 * ntiles);
-  uint32_t* totals = s.get<uint32_t>(256);
-  if (!counts || !totals) return BH_ENOMEM;
+  const int ntiles = (int)radix_tiles(n);
   uint64_t *ka = keys, *kb = keys_alt;
   uint32_t *va = vals, *vb = vals_alt;
   for (int shift = 0; shift < bits; shift += 8) {
@@ -497,6 +495,19 @@ int radix_sort(bh_ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int bits, u
   *keys_out = ka;
   *vals_out = va;
   return BH_OK;
+}
+
+int radix_sort(bh_ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int bits, uint64_t** keys_out,
+               uint32_t** vals_out, uint64_t* keys_alt, uint32_t* vals_alt) {
+  *keys_out = keys;
+  *vals_out = vals;
+  if (n <= 1) return BH_OK;
+  if (n > 0xffffffffll) return BH_EINVAL;
+  Scratch s(c);
+  uint32_t* counts = s.get<uint32_t>((size_t)256 * radix_tiles(n));
+  uint32_t* totals = s.get<uint32_t>(256);
+  if (!counts || !totals) return BH_ENOMEM;
+  return radix_sort_ws(c, keys, vals, n, bits, keys_out, vals_out, keys_alt, vals_alt, counts, totals);
 }
 
 // ---------------------------------------------------------------- gather

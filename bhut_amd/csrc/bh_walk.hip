@@ -220,22 +220,43 @@ static void launch_walk(bh_ctx* c, int mode, const bh_tree* t, int64_t m, const 
 }
 
 // Orders external targets along the tree's Morton curve (codes relative to
-// the source box, clamped) so each wavefront walks nearby targets.
+// the source box, clamped) so each wavefront walks nearby targets.  The
+// workspace holds two key and one value buffer of m entries and the radix
+// sort's counts (radix_tiles(m) * 256) and totals (256).
+static int order_targets_ws(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, uint32_t* torder,
+                            uint64_t* k0, uint64_t* k1, uint32_t* v1, uint32_t* counts, uint32_t* totals) {
+  int rc = morton_iota(c, tgt, m, t->dim, t->gbox, 1, k0, torder);
+  if (rc) return rc;
+  uint64_t* ko;
+  uint32_t* vo;
+  rc = radix_sort_ws(c, k0, torder, m, 21 * t->dim, &ko, &vo, k1, v1, counts, totals);
+  if (rc) return rc;
+  if (vo != torder) BH_HIP(hipMemcpyAsync(torder, vo, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+  return BH_OK;
+}
+
 int order_targets(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, uint32_t* torder) {
   if (m == 0) return BH_OK;
   Scratch s(c);
   uint64_t* k0 = s.get<uint64_t>(m);
   uint64_t* k1 = s.get<uint64_t>(m);
   uint32_t* v1 = s.get<uint32_t>(m);
-  if (!k0 || !k1 || !v1) return BH_ENOMEM;
-  int rc = morton_iota(c, tgt, m, t->dim, t->gbox, 1, k0, torder);
-  if (rc) return rc;
-  uint64_t* ko;
-  uint32_t* vo;
-  rc = radix_sort(c, k0, torder, m, 21 * t->dim, &ko, &vo, k1, v1);
-  if (rc) return rc;
-  if (vo != torder) BH_HIP(hipMemcpyAsync(torder, vo, m * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-  return BH_OK;
+  uint32_t* counts = s.get<uint32_t>((size_t)256 * radix_tiles(m));
+  uint32_t* totals = s.get<uint32_t>(256);
+  if (!k0 || !k1 || !v1 || !counts || !totals) return BH_ENOMEM;
+  return order_targets_ws(c, t, tgt, m, torder, k0, k1, v1, counts, totals);
+}
+
+static void launch_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, const uint32_t* torder, int64_t m,
+                        double theta, double eps2, double G, double* out, const int64_t* lptr, int32_t* lids,
+                        int64_t* stats) {
+  if (t->dim == 3) {
+    if (tgt) launch_walk<3, false>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+    else launch_walk<3, true>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+  } else {
+    if (tgt) launch_walk<2, false>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+    else launch_walk<2, true>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+  }
 }
 
 // Runs one walk mode.  tgt == nullptr: self mode (m == n).  For list fill,
@@ -244,7 +265,6 @@ int order_targets(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, uin
 int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t m, double theta,
               double eps, double G, double* out, const int64_t* lptr, int32_t* lids, int64_t* stats) {
   if (m == 0) return BH_OK;
-  const double eps2 = eps * eps;
   Scratch s(c);
   uint32_t* torder = nullptr;
   if (tgt) {
@@ -256,13 +276,83 @@ int walk_mode(bh_ctx* c, const bh_tree* t, int mode, const double* tgt, int64_t 
   static const char* names[] = {"k_walk_f64", "k_walk_f64_lcount", "k_walk_f64_lfill", "k_walk_f64_stats",
                                 "k_walk_f64_pot"};
   ProfScope ps(c, names[mode]);
-  if (t->dim == 3) {
-    if (tgt) launch_walk<3, false>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
-    else launch_walk<3, true>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
-  } else {
-    if (tgt) launch_walk<2, false>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
-    else launch_walk<2, true>(c, mode, t, m, tgt, torder, theta, eps2, G, out, lptr, lids, stats);
+  launch_mode(c, t, mode, tgt, torder, m, theta, eps * eps, G, out, lptr, lids, stats);
+  BH_LAUNCH_CHECK();
+  return BH_OK;
+}
+
+// ---------------------------------------------------------------- no allocation
+// Workspace of walk_targets_noalloc in one file-scope device array: the
+// Morton ordering of at most NOALLOC_TARGETS external targets (two key
+// buffers, the order and its radix partner, the radix sort's per-tile digit
+// counts and digit totals), 24 B per target + 1 KiB per 4096 targets + 1 KiB
+// = 24.25 MiB, held by every device the library runs on until the process
+// ends.  Every region is written before it is read, so nothing needs
+// clearing between calls; calls that overlap on one device share it and race.
+constexpr int64_t NOALLOC_TARGETS = 1 << 20;
+constexpr size_t NA_KEYS0 = 0;
+constexpr size_t NA_KEYS1 = NA_KEYS0 + 8 * NOALLOC_TARGETS;
+constexpr size_t NA_ORDER = NA_KEYS1 + 8 * NOALLOC_TARGETS;
+constexpr size_t NA_VALS1 = NA_ORDER + 4 * NOALLOC_TARGETS;
+constexpr size_t NA_COUNTS = NA_VALS1 + 4 * NOALLOC_TARGETS;
+constexpr size_t NA_TOTALS = NA_COUNTS + 4 * 256 * (NOALLOC_TARGETS / RADIX_TILE);
+constexpr size_t NA_BYTES = NA_TOTALS + 4 * 256;
+static_assert(NOALLOC_TARGETS % RADIX_TILE == 0 && NA_TOTALS % 256 == 0, "workspace layout");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex g_na_mu;
+static unsigned char* g_na_ptr[NA_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device`, which must be current: each
+// device holds its own copy of the array at its own address, and host code
+// passes this address, never the array's name, to kernels and copies.
+int noalloc_setup(int device) {
+  if (device < 0 || device >= NA_MAX_DEVICES) {
+    set_error("device %d beyond the %d devices the no-allocation workspace supports", device, NA_MAX_DEVICES);
+    return BH_ENODEV;
   }
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_ptr[device]) return BH_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+  if (!p) {
+    set_error("hipGetSymbolAddress(scratch) returned NULL on device %d", device);
+    return BH_EHIP;
+  }
+  g_na_ptr[device] = static_cast<unsigned char*>(p);
+  return BH_OK;
+}
+
+// The plain external-target walk of walk_mode(WALK_ACC) with its workspace
+// in `scratch`: no device allocation, no profiling events.
+int walk_targets_noalloc(bh_ctx* c, const bh_tree* t, const double* tgt, int64_t m, double theta, double eps,
+                         double G, double* out) {
+  if (m > NOALLOC_TARGETS) {
+    set_error("the no-allocation entry point takes at most %lld targets per call, got %lld",
+              (long long)NOALLOC_TARGETS, (long long)m);
+    return BH_EINVAL;
+  }
+  if (m == 0) return BH_OK;
+  unsigned char* base;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    base = c->device < NA_MAX_DEVICES ? g_na_ptr[c->device] : nullptr;
+  }
+  if (!base) {
+    set_error("no-allocation workspace not set up on device %d", c->device);
+    return BH_EHIP;
+  }
+  uint64_t* k0 = reinterpret_cast<uint64_t*>(base + NA_KEYS0);
+  uint64_t* k1 = reinterpret_cast<uint64_t*>(base + NA_KEYS1);
+  uint32_t* torder = reinterpret_cast<uint32_t*>(base + NA_ORDER);
+  uint32_t* v1 = reinterpret_cast<uint32_t*>(base + NA_VALS1);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(base + NA_COUNTS);
+  uint32_t* totals = reinterpret_cast<uint32_t*>(base + NA_TOTALS);
+  BH_TRY(order_targets_ws(c, t, tgt, m, torder, k0, k1, v1, counts, totals));
+  launch_mode(c, t, WALK_ACC, tgt, torder, m, theta, eps * eps, G, out, nullptr, nullptr, nullptr);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }

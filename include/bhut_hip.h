@@ -136,6 +136,19 @@ int bh_accel_targets(bh_tree *t, const double *targets, int64_t m, double theta,
 int bh_accel_targets_dev(bh_tree *t, const double *d_targets, int64_t m, double theta,
                          double softening, double G, double *d_out);
 
+/* bh_accel_targets_dev without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * targets' Morton ordering lives in a fixed workspace the library holds on
+ * every device it runs on (24.25 MiB, set up by bh_ctx_create), so m is at
+ * most 1048576 per call (BH_EINVAL above that: split larger queries).  Same
+ * results as bh_accel_targets_dev bit for bit.  Calls on one device share
+ * the workspace: calls that overlap on one device (other threads, other
+ * contexts' streams) race and must be serialised by the caller.  Enqueued
+ * on the context stream; returns at once. */
+int bh_accel_targets_dev_noalloc(bh_tree *t, const double *d_targets, int64_t m, double theta,
+                                 double softening, double G, double *d_out);
+
 /* ------------------------------------------- accelerations (FP32, chosen) */
 /* Same contract as bh_accel_self / bh_accel_targets, computed in FP32 on
  * coordinates relative to leaf and group centres, with a conservative group
