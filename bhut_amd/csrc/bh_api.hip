@@ -130,7 +130,7 @@ struct DeviceGuard {
 
 static void free_tree_arrays(bh_tree* t) {
   bh_ctx* c = t->ctx;
-  void* ps[] = {t->spart, t->perm,  t->start, t->count, t->first_child, t->parent, t->skip,
+  void* ps[] = {t->spart, t->perm,  t->start, t->count, t->first_child, t->parent, t->skip, t->ndepth,
                 t->leaf_order, t->center, t->half, t->mass, t->com, t->bmin, t->bmax, t->gbox,
                 t->nodes, t->spart32, t->pleaf, t->quad, t->quad32, t->d_sys_poff, t->d_sys_noff, t->d_gtab,
                 t->slot_node, t->d_lvl_sys, t->d_slot_off};
@@ -148,7 +148,8 @@ static int alloc_topology(bh_tree* t) {
   const size_t M = (size_t)t->M;
   if (!alloc_into(c, t->start, M) || !alloc_into(c, t->count, M) ||
       !alloc_into(c, t->first_child, M) || !alloc_into(c, t->parent, M) ||
-      !alloc_into(c, t->skip, M) || !alloc_into(c, t->leaf_order, (size_t)t->nleaves) ||
+      !alloc_into(c, t->skip, M) || !alloc_into(c, t->ndepth, M) ||
+      !alloc_into(c, t->leaf_order, (size_t)t->nleaves) ||
       !alloc_into(c, t->center, M * t->dim) || !alloc_into(c, t->half, M) ||
       !alloc_into(c, t->mass, M) || !alloc_into(c, t->com, M * t->dim) ||
       !alloc_into(c, t->bmin, M * t->dim) || !alloc_into(c, t->bmax, M * t->dim) ||
@@ -602,7 +603,7 @@ int bh_tree_import(bh_ctx* c, int64_t n, int64_t M, int32_t dim, int32_t leaf_si
          "NULL array");
   // every index the kernels will follow is checked here
   for (int64_t i = 0; i < n; i++) BH_REQ(perm[i] >= 0 && perm[i] < n, "perm[%lld] out of range", (long long)i);
-  std::vector<int32_t> skip(M, -1), par(M, -1), opento(M, -1);
+  std::vector<int32_t> skip(M, -1), par(M, -1), opento(M, -1), depth(M, 0);
   std::vector<NodeD> nodes(M);
   bool binary = true;
   for (int64_t k = 0; k < M; k++) {
@@ -620,6 +621,7 @@ int bh_tree_import(bh_ctx* c, int64_t n, int64_t M, int32_t dim, int32_t leaf_si
     for (int64_t i = 0; i < cc; i++) {
       skip[fc + i] = i > 0 ? (int32_t)(fc + i - 1) : skip[k];
       par[fc + i] = (int32_t)k;
+      depth[fc + i] = depth[k] + 1;
     }
     opento[k] = cc > 0 ? (int32_t)(fc + cc - 1) : skip[k];
   }
@@ -636,7 +638,7 @@ int bh_tree_import(bh_ctx* c, int64_t n, int64_t M, int32_t dim, int32_t leaf_si
     r.start = (int32_t)start[k];
     r.count = (int32_t)count[k];
     r.pad[0] = opento[k];
-    r.pad[1] = 0;
+    r.pad[1] = depth[k];
     if (first_child[k] == -1) leaves.push_back((int32_t)k);
   }
   std::sort(leaves.begin(), leaves.end(), [&](int32_t a, int32_t b) { return start[a] < start[b]; });
@@ -687,6 +689,7 @@ int bh_tree_import(bh_ctx* c, int64_t n, int64_t M, int32_t dim, int32_t leaf_si
   up(t->first_child, fc.data(), (size_t)M * 4);
   up(t->parent, par.data(), (size_t)M * 4);
   up(t->skip, skip.data(), (size_t)M * 4);
+  up(t->ndepth, depth.data(), (size_t)M * 4);
   up(t->leaf_order, leaves.data(), leaves.size() * 4);
   up(t->center, center, (size_t)M * dim * 8);
   up(t->half, half_size, (size_t)M * 8);
@@ -742,6 +745,7 @@ int bh_tree_clone(const bh_tree* src, bh_tree** out) {
   cp(t->first_child, src->first_child, M * 4);
   cp(t->parent, src->parent, M * 4);
   cp(t->skip, src->skip, M * 4);
+  cp(t->ndepth, src->ndepth, M * 4);
   cp(t->leaf_order, src->leaf_order, (size_t)t->nleaves * 4);
   cp(t->center, src->center, M * D * 8);
   cp(t->half, src->half, M * 8);

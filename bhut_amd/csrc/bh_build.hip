@@ -46,6 +46,7 @@ __global__ __launch_bounds__(256) void k_topology(int64_t nsys, int64_t L, int64
                                                   int32_t* __restrict__ first_child,
                                                   int32_t* __restrict__ parent,
                                                   int32_t* __restrict__ skip,
+                                                  int32_t* __restrict__ ndepth,
                                                   int32_t* __restrict__ leaf_order,
                                                   int32_t* __restrict__ slot_node) {
   for (int64_t gslot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gslot < nslots;
@@ -99,6 +100,7 @@ __global__ __launch_bounds__(256) void k_topology(int64_t nsys, int64_t L, int64
     count[gi] = (int32_t)c;
     parent[gi] = par >= 0 ? (int32_t)(par + S.noff) : -1;
     skip[gi] = sk >= 0 ? (int32_t)(sk + S.noff) : -1;
+    ndepth[gi] = k;
     if (c > L) {
       first_child[gi] = (int32_t)(1 + 2 * r + S.noff);
     } else {
@@ -256,7 +258,7 @@ int topology(bh_ctx* c, bh_tree* t) {
   }
   const int nb = (int)std::min<int64_t>(cdiv(slots, 256), 65536);
   hipLaunchKernelGGL(k_topology, dim3(nb), dim3(256), 0, c->stream, nsys, L, slots, d_info, d_tb, t->start,
-                     t->count, t->first_child, t->parent, t->skip, t->leaf_order, t->slot_node);
+                     t->count, t->first_child, t->parent, t->skip, t->ndepth, t->leaf_order, t->slot_node);
   BH_LAUNCH_CHECK();
   // the host tables of a forest must outlive the asynchronous copies
   if (nsys > 1) BH_HIP(hipStreamSynchronize(c->stream));
@@ -565,6 +567,7 @@ __global__ __launch_bounds__(256) void k_pack(int64_t M, const double* __restric
                                               const double* __restrict__ com,
                                               const int32_t* __restrict__ first_child,
                                               const int32_t* __restrict__ skip,
+                                              const int32_t* __restrict__ ndepth,
                                               const int32_t* __restrict__ start,
                                               const int32_t* __restrict__ count,
                                               NodeD* __restrict__ nodes) {
@@ -581,7 +584,7 @@ __global__ __launch_bounds__(256) void k_pack(int64_t M, const double* __restric
   r.start = start[k];
   r.count = count[k];
   r.pad[0] = r.fc >= 0 ? r.fc + 1 : r.skip;  // node popped first when opened
-  r.pad[1] = 0;
+  r.pad[1] = ndepth[k];
   nodes[k] = r;
 }
 
@@ -589,10 +592,10 @@ int pack_nodes(bh_ctx* c, bh_tree* t) {
   const unsigned nb = (unsigned)cdiv(t->M, 256);
   if (t->dim == 3)
     hipLaunchKernelGGL(k_pack<3>, dim3(nb), dim3(256), 0, c->stream, t->M, t->half, t->mass, t->com,
-                       t->first_child, t->skip, t->start, t->count, t->nodes);
+                       t->first_child, t->skip, t->ndepth, t->start, t->count, t->nodes);
   else
     hipLaunchKernelGGL(k_pack<2>, dim3(nb), dim3(256), 0, c->stream, t->M, t->half, t->mass, t->com,
-                       t->first_child, t->skip, t->start, t->count, t->nodes);
+                       t->first_child, t->skip, t->ndepth, t->start, t->count, t->nodes);
   BH_LAUNCH_CHECK();
   return BH_OK;
 }

@@ -61,7 +61,7 @@ struct __attribute__((aligned(64))) NodeD {
   int32_t fc;            // first child (-1 for leaves)
   int32_t skip;          // next node after this subtree (-1: walk ends)
   int32_t start, count;  // particle range in sorted order
-  int32_t pad[2];
+  int32_t pad[2];        // [0] node popped first when opened, [1] depth below the system root
 };
 static_assert(sizeof(NodeD) == 64, "NodeD layout");
 
@@ -165,6 +165,7 @@ struct bh_tree {
   int32_t* first_child = nullptr;
   int32_t* parent = nullptr;
   int32_t* skip = nullptr;
+  int32_t* ndepth = nullptr;      // depth below the node's system root (root 0)
   int32_t* leaf_order = nullptr;  // leaves left to right
   // depth tables of a built tree (NULL slot_node: imported or cloned): slot
   // (system, depth d, position) -> node or -1; per depth the number of

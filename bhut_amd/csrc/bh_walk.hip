@@ -8,12 +8,26 @@
 // potential over the same visits (bhut documents `bhut.potential`,
 // docs/getting-started.md:36, without implementing it).
 //
-// One lane per target, targets in Morton order so a wavefront walks similar
-// paths.  The reference's LIFO stack pops the right child first, i.e. visits
-// nodes in right-first preorder; each record carries the node that follows
-// its subtree in that order (`skip`) and the child popped first when it is
-// opened (`pad[0]` = first_child + 1), so the walk needs no stack and visits,
-// accepts and sums in exactly the reference's order.
+// One lane per target, targets in Morton order (self targets in sorted order,
+// external targets sorted on their Morton codes), 64 consecutive targets per
+// wavefront.  The reference's LIFO stack pops the right child first, i.e.
+// visits nodes in right-first preorder; each record carries the node that
+// follows its subtree in that order (`skip`), the child popped first when it
+// is opened (`pad[0]`) and its depth (`pad[1]`).
+//
+// The wavefront walks the UNION of its targets' walks in lockstep: one node
+// at a time, in that preorder.  A target visits a node exactly when it opened
+// every ancestor of the node, so every target's own visits are a subsequence
+// of the union and each lane still tests, accepts and sums in its own pop
+// order, bit for bit.  Per lane, `follow` is the depth down to which the
+// lane opened the current path (-1: not walking this system): at a node of
+// depth d it is min(follow, d), and the lane visits the node iff follow == d;
+// opening the node sets follow = d + 1.  The wavefront goes down into the
+// children iff some lane opened the node, else on to `skip`.  Node records
+// and leaf particles are read once per wavefront (uniform scalar loads) and
+// broadcast to the lanes; a leaf's particle pairs run on every visiting lane
+// at once, where the per-lane walk of the same lists ran only the lanes that
+// happened to be at a leaf.
 #include "bh_common.h"
 
 namespace bh {
@@ -31,158 +45,184 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
                                               const int32_t* __restrict__ sys_poff,
                                               const int32_t* __restrict__ sys_noff, int64_t nsys) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  int64_t orig;
-  double tx, ty, tz = 0.0;
-  if (SELF) {
-    orig = perm[t];
-    const double4 p = spart[t];
-    tx = p.x;
-    ty = p.y;
-    tz = p.z;
-  } else {
-    orig = torder[t];
-    tx = tgt[orig * DIM + 0];
-    ty = tgt[orig * DIM + 1];
-    if (DIM == 3) tz = tgt[orig * DIM + 2];
+  // the whole wavefront past the end (its first target index is uniform)
+  if (blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~(WAVE - 1)) >= m) return;
+  const bool live = t < m;
+  int64_t orig = 0;
+  double tx = 0.0, ty = 0.0, tz = 0.0;
+  if (live) {
+    if (SELF) {
+      orig = perm[t];
+      const double4 p = spart[t];
+      tx = p.x;
+      ty = p.y;
+      tz = p.z;
+    } else {
+      orig = torder[t];
+      tx = tgt[orig * DIM + 0];
+      ty = tgt[orig * DIM + 1];
+      if (DIM == 3) tz = tgt[orig * DIM + 2];
+    }
   }
   double ax = 0.0, ay = 0.0, az = 0.0, phi = 0.0;
   int64_t nl = 0, wpos = 0;
   int64_t c_pop = 0, c_test = 0, c_mono = 0, c_leaf = 0, c_pair = 0;
-  if (MODE == WALK_LFILL) wpos = lptr[orig];
-  // root of the target's own system (a forest walks each system alone)
-  int v = (SELF && nsys > 1) ? sys_noff[find_sys(sys_poff, nsys, t)] : 0;
-  while (v >= 0) {
-    const NodeD nd = nodes[v];
-    if (MODE == WALK_STATS) c_pop++;
-    if (nd.m == 0.0) {  // bh.py:512-513
-      v = nd.skip;
-      continue;
-    }
-    if (nd.fc < 0) {  // leaf: direct particle-particle sum (bh.py:518-522)
-      if (MODE == WALK_LCOUNT) nl++;
-      if (MODE == WALK_LFILL) lids[wpos++] = v;
-      if (MODE == WALK_STATS) {
-        c_leaf++;
-        c_pair += nd.count;
-        if (SELF && t >= nd.start && t < nd.start + nd.count) c_pair--;
+  if (MODE == WALK_LFILL && live) wpos = lptr[orig];
+  // a forest walks each system alone: the lanes of one system at a time
+  const int64_t sys = (SELF && nsys > 1 && live) ? find_sys(sys_poff, nsys, t) : 0;
+  uint64_t todo = __ballot(live);
+  while (todo) {
+    const int lead = __ffsll((unsigned long long)todo) - 1;
+    const int64_t s = __shfl(sys, lead);
+    const bool part = live && sys == s;
+    todo &= ~__ballot(part);
+    int follow = part ? 0 : -1;
+    int v = (SELF && nsys > 1) ? sys_noff[s] : 0;
+    while (v >= 0) {
+      v = __builtin_amdgcn_readfirstlane(v);
+      const NodeD nd = nodes[v];
+      const int dv = nd.pad[1];
+      follow = min(follow, dv);
+      const bool mine = follow == dv;
+      if (MODE == WALK_STATS && mine) c_pop++;
+      if (nd.m == 0.0) {  // bh.py:512-513
+        v = nd.skip;
+        continue;
       }
-      if (MODE == WALK_ACC) {
-        const int64_t j0 = nd.start, j1 = nd.start + nd.count;
-        double px = 0.0, py = 0.0, pz = 0.0;
-        bool zdiv = false;
-        // Numba loop (bh.py:249-278): r = sqrt(r2 + eps^2); 1/r; r^-3
-        for (int64_t j = j0; j < j1; j++) {
-          if (SELF && j == t) continue;
-          const double4 q = spart[j];
-          const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
-          double rsq = dx * dx;
-          rsq = rsq + dy * dy;
-          if (DIM == 3) rsq = rsq + dz * dz;
-          rsq = rsq + eps2;
-          const double r = sqrt(rsq);
-          if (r == 0.0) {  // Numba raises ZeroDivisionError here
-            zdiv = true;
-            break;
+      if (nd.fc < 0) {  // leaf: direct particle-particle sum (bh.py:518-522)
+        if (mine) {
+          if (MODE == WALK_LCOUNT) nl++;
+          if (MODE == WALK_LFILL) lids[wpos++] = v;
+          if (MODE == WALK_STATS) {
+            c_leaf++;
+            c_pair += nd.count;
+            if (SELF && t >= nd.start && t < nd.start + nd.count) c_pair--;
           }
-          const double rinv = 1.0 / r;
-          const double ric = rinv * rinv * rinv;
-          const double mag = G * q.w * ric;
-          px = px + mag * dx;
-          py = py + mag * dy;
-          if (DIM == 3) pz = pz + mag * dz;
         }
-        if (zdiv) {
-          // the whole leaf again with the pure-Python loop (bh.py:384-425):
-          // G m / (r2 + eps^2)^1.5, zero where r2 + eps^2 == 0
-          px = py = pz = 0.0;
-          for (int64_t j = j0; j < j1; j++) {
-            if (SELF && j == t) continue;
+        const int j0 = nd.start, j1 = nd.start + nd.count;
+        if (MODE == WALK_ACC) {
+          double px = 0.0, py = 0.0, pz = 0.0;
+          bool zdiv = false;
+          // Numba loop (bh.py:249-278): r = sqrt(r2 + eps^2); 1/r; r^-3.
+          // The sources are the same for every lane (scalar loads).
+          for (int j = j0; j < j1; j++) {
             const double4 q = spart[j];
-            const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+            if (mine && !(SELF && j == t)) {
+              const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+              double rsq = dx * dx;
+              rsq = rsq + dy * dy;
+              if (DIM == 3) rsq = rsq + dz * dz;
+              rsq = rsq + eps2;
+              const double r = sqrt(rsq);
+              zdiv |= r == 0.0;  // Numba raises ZeroDivisionError here
+              const double rinv = 1.0 / r;
+              const double ric = rinv * rinv * rinv;
+              const double mag = G * q.w * ric;
+              px = px + mag * dx;
+              py = py + mag * dy;
+              if (DIM == 3) pz = pz + mag * dz;
+            }
+          }
+          if (zdiv) {
+            // the whole leaf again with the pure-Python loop (bh.py:384-425):
+            // G m / (r2 + eps^2)^1.5, zero where r2 + eps^2 == 0
+            px = py = pz = 0.0;
+            for (int j = j0; j < j1; j++) {
+              if (SELF && j == t) continue;
+              const double4 q = spart[j];
+              const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+              double r2 = dx * dx;
+              r2 = r2 + dy * dy;
+              if (DIM == 3) r2 = r2 + dz * dz;
+              const double r2s = r2 + eps2;
+              if (r2s == 0.0) {
+                px = px + 0.0;
+                py = py + 0.0;
+                pz = pz + 0.0;
+                continue;
+              }
+              const double f = (G * q.w) / (r2s * sqrt(r2s));
+              px = px + f * dx;
+              py = py + f * dy;
+              if (DIM == 3) pz = pz + f * dz;
+            }
+          }
+          if (mine) {
+            ax = ax + px;
+            ay = ay + py;
+            if (DIM == 3) az = az + pz;
+          }
+        }
+        if (MODE == WALK_POT) {
+          // softened pair potential -G m / sqrt(r^2 + eps^2), zero where the
+          // softened distance is zero; leaf partial added to the total
+          double pp = 0.0;
+          for (int j = j0; j < j1; j++) {
+            const double4 q = spart[j];
+            if (mine && !(SELF && j == t)) {
+              const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+              double r2 = dx * dx;
+              r2 = r2 + dy * dy;
+              if (DIM == 3) r2 = r2 + dz * dz;
+              const double r2s = r2 + eps2;
+              if (r2s != 0.0) pp = pp - (G * q.w) / sqrt(r2s);
+            }
+          }
+          if (mine) phi = phi + pp;
+        }
+        v = nd.skip;
+        continue;
+      }
+      bool open = false;
+      if (mine) {
+        // opening test (bh.py:586-601): d = ||t - com|| as OpenBLAS ddot
+        // forms it (x*x, then fma for y and z), open when d == 0 or
+        // size/d >= theta
+        if (MODE == WALK_STATS) c_test++;
+        const double ux = tx - nd.cx, uy = ty - nd.cy, uz = tz - nd.cz;
+        double d2 = ux * ux;
+        d2 = fma(uy, uy, d2);
+        if (DIM == 3) d2 = fma(uz, uz, d2);
+        const double dist = sqrt(d2);
+        open = (dist == 0.0) || (nd.size / dist >= theta);
+        if (open) {
+          follow = dv + 1;
+        } else {
+          // monopole (kernels.py:57-69 via acc_monopole_node)
+          if (MODE == WALK_LCOUNT) nl++;
+          if (MODE == WALK_LFILL) lids[wpos++] = v;
+          if (MODE == WALK_STATS) c_mono++;
+          if (MODE == WALK_POT) {
+            const double dx = nd.cx - tx, dy = nd.cy - ty, dz = nd.cz - tz;
             double r2 = dx * dx;
             r2 = r2 + dy * dy;
             if (DIM == 3) r2 = r2 + dz * dz;
             const double r2s = r2 + eps2;
-            if (r2s == 0.0) {
-              px = px + 0.0;
-              py = py + 0.0;
-              pz = pz + 0.0;
-              continue;
+            if (r2s != 0.0) phi = phi - (G * nd.m) / sqrt(r2s);
+          }
+          if (MODE == WALK_ACC) {
+            const double dx = nd.cx - tx, dy = nd.cy - ty, dz = nd.cz - tz;
+            double r2 = dx * dx;
+            r2 = r2 + dy * dy;
+            if (DIM == 3) r2 = r2 + dz * dz;
+            const double r2s = r2 + eps2;
+            if (r2s != 0.0) {
+              const double f = (G * nd.m) / (r2s * sqrt(r2s));
+              ax = ax + f * dx;
+              ay = ay + f * dy;
+              if (DIM == 3) az = az + f * dz;
+            } else {
+              ax = ax + 0.0;
+              ay = ay + 0.0;
+              az = az + 0.0;
             }
-            const double f = (G * q.w) / (r2s * sqrt(r2s));
-            px = px + f * dx;
-            py = py + f * dy;
-            if (DIM == 3) pz = pz + f * dz;
           }
         }
-        ax = ax + px;
-        ay = ay + py;
-        if (DIM == 3) az = az + pz;
       }
-      if (MODE == WALK_POT) {
-        // softened pair potential -G m / sqrt(r^2 + eps^2), zero where the
-        // softened distance is zero; leaf partial added to the total
-        double pp = 0.0;
-        for (int64_t j = nd.start; j < nd.start + nd.count; j++) {
-          if (SELF && j == t) continue;
-          const double4 q = spart[j];
-          const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
-          double r2 = dx * dx;
-          r2 = r2 + dy * dy;
-          if (DIM == 3) r2 = r2 + dz * dz;
-          const double r2s = r2 + eps2;
-          if (r2s != 0.0) pp = pp - (G * q.w) / sqrt(r2s);
-        }
-        phi = phi + pp;
-      }
-      v = nd.skip;
-      continue;
+      v = __ballot(open) ? nd.pad[0] : nd.skip;
     }
-    // opening test (bh.py:586-601): d = ||t - com|| as OpenBLAS ddot forms
-    // it (x*x, then fma for y and z), open when d == 0 or size/d >= theta
-    if (MODE == WALK_STATS) c_test++;
-    const double ux = tx - nd.cx, uy = ty - nd.cy, uz = tz - nd.cz;
-    double d2 = ux * ux;
-    d2 = fma(uy, uy, d2);
-    if (DIM == 3) d2 = fma(uz, uz, d2);
-    const double dist = sqrt(d2);
-    const bool open = (dist == 0.0) || (nd.size / dist >= theta);
-    if (open) {
-      v = nd.pad[0];
-      continue;
-    }
-    // monopole (kernels.py:57-69 via acc_monopole_node)
-    if (MODE == WALK_LCOUNT) nl++;
-    if (MODE == WALK_LFILL) lids[wpos++] = v;
-    if (MODE == WALK_STATS) c_mono++;
-    if (MODE == WALK_POT) {
-      const double dx = nd.cx - tx, dy = nd.cy - ty, dz = nd.cz - tz;
-      double r2 = dx * dx;
-      r2 = r2 + dy * dy;
-      if (DIM == 3) r2 = r2 + dz * dz;
-      const double r2s = r2 + eps2;
-      if (r2s != 0.0) phi = phi - (G * nd.m) / sqrt(r2s);
-    }
-    if (MODE == WALK_ACC) {
-      const double dx = nd.cx - tx, dy = nd.cy - ty, dz = nd.cz - tz;
-      double r2 = dx * dx;
-      r2 = r2 + dy * dy;
-      if (DIM == 3) r2 = r2 + dz * dz;
-      const double r2s = r2 + eps2;
-      if (r2s != 0.0) {
-        const double f = (G * nd.m) / (r2s * sqrt(r2s));
-        ax = ax + f * dx;
-        ay = ay + f * dy;
-        if (DIM == 3) az = az + f * dz;
-      } else {
-        ax = ax + 0.0;
-        ay = ay + 0.0;
-        az = az + 0.0;
-      }
-    }
-    v = nd.skip;
   }
+  if (!live) return;
   if (MODE == WALK_ACC) {
     out[orig * DIM + 0] = ax;
     out[orig * DIM + 1] = ay;
