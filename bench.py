@@ -141,9 +141,9 @@ def roofline_f32(prof, steps, stats):
     kstep = kms / max(steps, 1)
     flops = FLOPS_PER_INTERACTION * int(stats[1]) + FLOPS_PER_QUAD_CELL * int(stats[0])
     achieved = flops / (kstep * 1e-3) / 1e12 if kstep > 0 else None
-    # compute-bound: priced against the FP32 compute figure; the work runs on
-    # the vector ALUs (nothing here is a dense contraction for MFMA)
-    return {"bound": "mfma", "compute_unit": COMPUTE_UNIT, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+    # compute-bound on the vector ALUs (nothing here is a dense contraction
+    # for MFMA): priced against the FP32 vector figure
+    return {"bound": "valu", "compute_unit": COMPUTE_UNIT, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS if achieved else None, "traffic": None,
             "kernel": "k_group_eval_f32", "kernel_avg_ms": kms / max(launches, 1),
@@ -166,55 +166,83 @@ def pair_loop_ceiling():
         return None
 
 
-def cpu_baseline(x, m, theta, eps, leaf, budget_s):
-    """Oracle build + accelerations on a bounded random sample of targets,
-    scaled to particle-accelerations/s for the whole job."""
-    from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    n = len(x)
-    t0 = time.perf_counter()
-    tree = O.build(x, m, leaf)
-    t_build = time.perf_counter() - t0
-    rng = np.random.default_rng(123)
-    order = rng.permutation(n)
-    done, t_acc = 0, 0.0
-    chunk = 2000
-    while done < n and t_build + t_acc < budget_s:
-        sel = order[done:done + chunk]
-        t0 = time.perf_counter()
-        O.accelerations(tree, x, m, x[sel], softening=eps, theta=theta, nthreads=threads)
-        t_acc += time.perf_counter() - t0
-        done += len(sel)
-        chunk = min(chunk * 2, 200_000)
-    per_target = t_acc / done
-    t_total = t_build + per_target * n
-    # the serial loop of the reference (one target after another on one
-    # core), on a smaller sample of the same targets: about a tenth of the
-    # budget
-    done1, t_acc1 = 0, 0.0
-    while done1 < n and t_acc1 < 0.1 * budget_s:
-        sel = order[done1:done1 + 500]
-        t0 = time.perf_counter()
-        O.accelerations(tree, x, m, x[sel], softening=eps, theta=theta, nthreads=1)
-        t_acc1 += time.perf_counter() - t0
-        done1 += len(sel)
-    t_total1 = t_build + t_acc1 / done1 * n
-    cpu = "unknown"
+def host_cpus():
+    """(threads to use, description): the CPUs this process may run on
+    (affinity), capped by the cgroup CPU quota and by OMP_NUM_THREADS when
+    either is set (the host a GPU is leased with grants a share of its
+    cores), with the CPU model."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
     try:
-        cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, int(int(q) // int(per)))
     except Exception:
         pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    omp = int(omp) if omp and omp.isdigit() else None
+    use = min(v for v in (aff, quota, omp) if v)
+    model = "unknown"
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        pass
+    return use, {"nproc": aff, "cgroup_cpu_quota": quota, "omp_num_threads": omp, "threads_used": use,
+                 "model": model}
+
+
+def cpu_baseline(x, m, theta, eps, leaf, budget_s):
+    """Oracle build + self accelerations on a bounded random sample of
+    targets, each timed in 3 runs after a warm-up (median kept), scaled to
+    particle-accelerations/s for the whole job."""
+    from oracle import oracle as O
+    threads, host = host_cpus()
+    n = len(x)
+    rng = np.random.default_rng(123)
+    order = rng.permutation(n)
+
+    def median_of(fn, runs=3):
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ts
+
+    tree = O.build(x, m, leaf)  # warm-up
+    t_build, builds = median_of(lambda: O.build(x, m, leaf))
+    # sample size: a probe of 2000 targets sets it so the three timed runs
+    # take about three quarters of the budget
+    probe = np.sort(order[:2000])
+    t0 = time.perf_counter()
+    O.accelerations_subset(tree, x, m, probe, softening=eps, theta=theta, nthreads=threads)
+    per = (time.perf_counter() - t0) / len(probe)
+    k = int(min(n, max(2000, 0.25 * budget_s / per)))
+    sel = np.sort(order[:k])
+    t_acc, accs = median_of(lambda: O.accelerations_subset(tree, x, m, sel, softening=eps, theta=theta,
+                                                           nthreads=threads))
+    t_total = t_build + t_acc / k * n
+    # the serial loop of the reference (one target after another on one
+    # core): a tenth of the budget over the three runs
+    k1 = int(min(n, max(200, 0.03 * budget_s / (per * threads))))
+    sel1 = np.sort(order[:k1])
+    t_acc1, _ = median_of(lambda: O.accelerations_subset(tree, x, m, sel1, softening=eps, theta=theta,
+                                                         nthreads=1))
+    t_total1 = t_build + t_acc1 / k1 * n
     return {
         "value": n / t_total,
         "unit": "particle-accels/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle build of all {n} particles ({t_build:.2f} s, 1 thread) + accelerations of {done} "
-                  f"random targets ({t_acc:.2f} s, {threads} OpenMP threads), extrapolated to {n} targets; "
-                  f"host CPU {cpu}",
+        "host": host,
+        "sample": f"oracle build of all {n} particles (median of 3: {t_build:.2f} s, 1 thread) + self "
+                  f"accelerations of {k} random particles (median of 3: {t_acc:.2f} s, runs "
+                  f"{', '.join(f'{v:.2f}' for v in accs)} s, {threads} OpenMP threads), extrapolated to {n} "
+                  f"targets; host CPU {host['model']}, {host['nproc']} CPUs in the affinity mask, "
+                  f"cgroup quota {host['cgroup_cpu_quota']}, OMP_NUM_THREADS {host['omp_num_threads']}",
         "single_thread": {"value": n / t_total1, "unit": "particle-accels/s", "cores": 1,
-                          "sample": f"the same build + {done1} random targets ({t_acc1:.2f} s, 1 thread), "
-                                    f"extrapolated to {n} targets"},
+                          "sample": f"the same build + {k1} random particles (median of 3: {t_acc1:.2f} s, "
+                                    f"1 thread), extrapolated to {n} targets"},
     }
 
 
@@ -300,7 +328,7 @@ def workload_plummer(args, env):
         kavg = kms / max(launches, 1)
         flops = FLOPS_PER_INTERACTION * inter64
         achieved = flops / (kavg * 1e-3) / 1e12 if kavg > 0 else None
-        roof = {"bound": "mfma", "compute_unit": COMPUTE_UNIT, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+        roof = {"bound": "valu", "compute_unit": COMPUTE_UNIT, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS if achieved else None, "traffic": None, "kernel": "k_walk_f64",
                 "kernel_avg_ms": kavg, "flops_per_launch": flops, "interactions_per_launch": inter64}
@@ -313,6 +341,9 @@ def workload_plummer(args, env):
         tpath = os.path.join(ROOT, "profiles", "plummer1m_group_eval_traffic.json")
         if N == 1_000_000 and leaf == 32 and opts.group_size in (0, 64) and os.path.exists(tpath):
             roof["traffic"] = json.load(open(tpath)).get("hbm_bytes_per_launch")
+            roof["traffic_source"] = ("read from profiles/plummer1m_group_eval_traffic.json: rocprofv3 --pmc "
+                                      "FETCH_SIZE and WRITE_SIZE passes of this workload, each scaled by the "
+                                      "counter's reading for a copy of known size; not measured in this run")
     line = base_line(args, N / (ms * 1e-3), ms, "fp32" if head == "fp32" else "fp64",
                      {"workload": "plummer_self_accelerations", "n": N, "theta": theta, "softening": eps,
                       "leaf_size": leaf, "G": G, "seed": 1, "step": "tree build + self accelerations", "form": head})
@@ -335,8 +366,14 @@ def workload_plummer(args, env):
         "group_size": opts.group_size, "stack_cap": opts.stack_cap, "kahan": opts.kahan,
         "multipole": opts.multipole, "autotune": tuning}
     if head == "fp32" and "fp64" in res:
+        launches, kms = res["fp64"]["prof"].get("k_walk_f64", (0, 0.0))
+        kavg = kms / max(launches, 1)
+        ach64 = FLOPS_PER_INTERACTION * inter64 / (kavg * 1e-3) / 1e12 if kavg > 0 else None
         line["plain_fp64"] = {"value": N / (res["fp64"]["ms"] * 1e-3), "ms_per_step": res["fp64"]["ms"],
                               "interactions_per_launch": inter64,
+                              "roofline": {"bound": "valu", "kernel": "k_walk_f64", "kernel_avg_ms": kavg,
+                                           "achieved": ach64, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                           "frac": ach64 / PEAK_FP64_TFLOPS if ach64 else None},
                               "kernels_ms_per_step": {k: v[1] / args.steps for k, v in res["fp64"]["prof"].items()}}
         line["speedup_fp32_vs_plain"] = res["fp64"]["ms"] / ms
 

@@ -613,6 +613,23 @@ void orc_accel(const orc_tree *t, const double *src_pos, const double *src_mass,
   }
 }
 
+/* orc_accel in self mode for the particles idx[0..k) only: particle idx[q]
+ * is the target and skips itself (bh.py:475-480 for i = idx[q]); out is
+ * (k, dim). */
+void orc_accel_subset(const orc_tree *t, const double *src_pos, const double *src_mass,
+                      const int64_t *idx, int64_t k, double eps, double theta, double G,
+                      int leaf_formula, int nthreads, double *out) {
+  const int dim = (int)t->dim;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t q = 0; q < k; q++) {
+    walk_out w;
+    memset(&w, 0, sizeof(w));
+    w.acc = out + q * dim;
+    walk(t, src_pos + idx[q] * dim, idx[q], src_pos, src_mass, eps, theta, G, leaf_formula, &w);
+  }
+}
+
 /* Potential over the same walk as orc_accel: accepted nodes and leaf partial
  * sums added in pop order. */
 void orc_potential(const orc_tree *t, const double *src_pos, const double *src_mass,

@@ -70,6 +70,9 @@ void orc_accel(const orc_tree *t, const double *src_pos, const double *src_mass,
                const double *tgt, int64_t m, int self_mode, double eps,
                double theta, double G, int leaf_formula, int nthreads,
                double *out);
+void orc_accel_subset(const orc_tree *t, const double *src_pos, const double *src_mass,
+                      const int64_t *idx, int64_t k, double eps, double theta, double G,
+                      int leaf_formula, int nthreads, double *out);
 
 /* Softened potential -G sum m / sqrt(r2 + eps2) over the same walk
  * (accepted nodes and leaf partial sums in pop order, self skipped in

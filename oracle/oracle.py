@@ -74,6 +74,7 @@ def lib():
         L.orc_should_refit.restype = ci
         L.orc_should_refit.argtypes = [ctypes.POINTER(_OrcTree), vp, i64, ci]
         L.orc_accel.argtypes = [ctypes.POINTER(_OrcTree), vp, vp, vp, i64, ci, dbl, dbl, dbl, ci, ci, vp]
+        L.orc_accel_subset.argtypes = [ctypes.POINTER(_OrcTree), vp, vp, vp, i64, dbl, dbl, dbl, ci, ci, vp]
         L.orc_potential.argtypes = [ctypes.POINTER(_OrcTree), vp, vp, vp, i64, ci, dbl, dbl, dbl, ci, vp]
         L.orc_target_list.restype = i64
         L.orc_target_list.argtypes = [ctypes.POINTER(_OrcTree), vp, dbl, vp, i64]
@@ -222,6 +223,21 @@ def accelerations(tree: OracleTree, src_pos, src_mass, targets=None, *, softenin
     st = tree._struct()
     lib().orc_accel(ctypes.byref(st), _p(src_pos), _p(src_mass), _p(tgt), len(tgt), int(self_mode),
                     float(softening), float(theta), float(G), int(leaf_formula), int(nthreads), _p(out))
+    return out
+
+
+def accelerations_subset(tree: OracleTree, src_pos, src_mass, idx, *, softening=0.0, theta=0.5, G=1.0,
+                         leaf_formula=LEAF_NUMBA, nthreads=0):
+    """Self accelerations of the particles idx only (each skips itself, as
+    in bh.py:475-480), (len(idx), dim)."""
+    src_pos, src_mass = _f64(src_pos), _f64(src_mass)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    if len(idx) and (idx.min() < 0 or idx.max() >= len(src_pos)):
+        raise IndexError("idx out of range")
+    out = np.zeros((len(idx), src_pos.shape[1]))
+    st = tree._struct()
+    lib().orc_accel_subset(ctypes.byref(st), _p(src_pos), _p(src_mass), _p(idx), len(idx), float(softening),
+                           float(theta), float(G), int(leaf_formula), int(nthreads), _p(out))
     return out
 
 

@@ -374,11 +374,34 @@ def test_full_size_plummer_properties(native):
         np.testing.assert_array_equal(ids[ptr[i]:ptr[i + 1]], O.target_list(full, x[i], 0.5))
     acc = dt.accel_self(0.5, 0.01, 1.0)
     sub = rng.choice(n, 2000, replace=False)
+    # the self walk of these particles (each skipping itself) in the oracle
+    acc_close(acc[sub], O.accelerations_subset(full, x, m, sub, softening=0.01, theta=0.5))
     ref = O.accelerations(full, x, m, x[sub], softening=0.01, theta=0.5)
-    # external walk of these points = self walk apart from the self pair,
-    # which is exactly zero for eps > 0 only after dx = 0; compare with
-    # direct sums instead for the self targets
-    d = native.direct(x, m, x[sub], softening=0.01)
-    err = np.linalg.norm(acc[sub] - d, axis=1) / np.linalg.norm(d, axis=1)
-    assert np.sqrt(np.mean(err ** 2)) < 2e-2
     acc_close(native.DeviceTree.accel_targets(dt, x[sub], 0.5, 0.01, 1.0), ref)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_direct_sum_matches_oracle(native, dim):
+    """The FP64 direct sum (k_direct), the accuracy reference of every FP32
+    test and of the bench line, against the oracle's direct sum within 1e-12:
+    self (j != i) and external targets, softened and unsoftened with a
+    coincident pair (that pair is skipped, r^2 + eps^2 == 0), potential too.
+    Its chunked summation differs from the oracle's sequential one only in
+    rounding."""
+    rng = np.random.default_rng(50 + dim)
+    n = 20_000
+    x = rng.standard_normal((n, dim))
+    x[7] = x[3]  # coincident particles
+    m = rng.uniform(0.1, 2.0, n)
+    tg = rng.uniform(-3, 3, (3000, dim))
+    tg[0] = x[11]
+    for eps in (0.0, 0.02):
+        acc, pot = native.direct(x, m, softening=eps, potential=True)
+        acc_close(acc, O.direct(x, m, softening=eps))
+        np.testing.assert_allclose(pot, O.direct_potential(x, m, softening=eps), rtol=1e-12)
+        acc, pot = native.direct(x, m, tg, softening=eps, potential=True)
+        acc_close(acc, O.direct(x, m, tg, softening=eps))
+        np.testing.assert_allclose(pot, O.direct_potential(x, m, tg, softening=eps), rtol=1e-12)
+    # theta = 0 walk == direct sum (bh.py:600-601)
+    dt = native.DeviceTree.build(x, m, 32)
+    acc_close(dt.accel_self(0.0, 0.02, 1.0), O.direct(x, m, softening=0.02))

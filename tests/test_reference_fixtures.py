@@ -213,3 +213,18 @@ def test_reference_pop_order_is_right_first_preorder():
         assert np.all(np.diff(rf[lst]) > 0), i
         left_sorted += bool(np.all(np.diff(lf[lst]) > 0))
     assert left_sorted == 0
+
+
+def test_subset_self_walk_equals_full_self_walk():
+    """O.accelerations_subset (self mode on chosen particles, each skipping
+    itself) returns the rows of the full self walk bit for bit, which the
+    reference fixtures pin."""
+    g = np.load(os.path.join(GOLD, "ref_gauss2000.npz"), allow_pickle=False)
+    x, m = g["pos"], g["mass"]
+    t = O.build(x, m, 8)
+    idx = np.random.default_rng(3).choice(len(x), 150, replace=False)
+    for eps in (0.0, 0.01):
+        full = O.accelerations(t, x, m, softening=eps, theta=0.5)
+        bits_equal(O.accelerations_subset(t, x, m, idx, softening=eps, theta=0.5), full[idx])
+    bits_equal(O.accelerations_subset(t, x, m, np.arange(96), softening=0.01, theta=0.5),
+               g["L8_t0.5_e0.01_acc_numba"])
