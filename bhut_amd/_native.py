@@ -62,6 +62,7 @@ _SIGS = {
     "bh_accel_targets": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp]),
     "bh_accel_targets_dev": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp]),
     "bh_accel_targets_dev_noalloc": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp]),
+    "bh_accel_targets_streamed": (ctypes.c_int, [vp, vp, i64, i64, dbl, dbl, dbl, i32, vp, vp]),
     "bh_accel_f32_self": (ctypes.c_int, [vp, dbl, dbl, dbl, vp, vp, vp]),
     "bh_accel_f32_self_dev": (ctypes.c_int, [vp, dbl, dbl, dbl, vp, vp, vp]),
     "bh_accel_f32_targets": (ctypes.c_int, [vp, vp, i64, dbl, dbl, dbl, vp, vp, vp]),
@@ -379,6 +380,21 @@ class DeviceTree:
         if len(tgt):
             check(load().bh_accel_targets(self.handle, ptr(tgt), len(tgt), float(theta), float(softening),
                                           float(G), ptr(out)))
+        return out
+
+    def accel_targets_streamed(self, targets: np.ndarray, theta: float, softening: float, G: float, *,
+                               precision: str = "fp64", chunk: int = 0, opts: F32Opts | None = None) -> np.ndarray:
+        """Accelerations at host targets streamed through the device in
+        chunks of `chunk` targets on two streams (0: 2**20); bit-identical
+        to accel_targets / accel_f32 on all targets at once."""
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"precision must be 'fp64' or 'fp32', got '{precision}'")
+        tgt = _rows(targets, "targets", None, self.dim)
+        out = np.empty_like(tgt)
+        if len(tgt):
+            check(load().bh_accel_targets_streamed(self.handle, ptr(tgt), len(tgt), int(chunk), float(theta),
+                                                   float(softening), float(G), 64 if precision == "fp64" else 32,
+                                                   ctypes.byref(opts) if opts is not None else None, ptr(out)))
         return out
 
     def accel_f32(self, targets, theta: float, softening: float, G: float, opts: F32Opts | None = None,

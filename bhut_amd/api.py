@@ -175,8 +175,11 @@ class Tree:
 
     def accelerations(self, targets: Optional[ArrayLike] = None, *, theta: float = 0.5,
                       softening: Union[float, ArrayLike] = 0.0, G: float = 1.0,
-                      precision: str = "fp64") -> np.ndarray:
-        """api.py:608-688."""
+                      precision: str = "fp64", chunk_size: Optional[int] = None) -> np.ndarray:
+        """api.py:608-688.  chunk_size streams external targets through the
+        device in chunks of that many targets (the counterpart of the
+        reference's Dask target chunks, api.py:690-731); the result equals
+        the unchunked call bit for bit."""
         if not self._is_built:
             raise ValueError("Tree must be built before evaluating accelerations")
         if targets is None:
@@ -194,7 +197,7 @@ class Tree:
         if theta < 0:
             raise ValueError(f"theta must be non-negative, got {theta}")
         return barnes_hut_accelerations_targets(self._tree, self.positions, self.masses, targets, softening, theta,
-                                                G, precision=precision)
+                                                G, precision=precision, chunk_size=chunk_size)
 
     def potential(self, targets: Optional[ArrayLike] = None, *, theta: float = 0.5, softening: float = 0.0,
                   G: float = 1.0) -> np.ndarray:

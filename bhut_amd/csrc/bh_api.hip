@@ -116,17 +116,6 @@ static void prof_reset(bh_ctx* c) {
   c->prof_recs.clear();
 }
 
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-    if (prev != dev) (void)hipSetDevice(dev);
-  }
-  ~DeviceGuard() {
-    int cur = -1;
-    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-  }
-};
 
 static void free_tree_arrays(bh_tree* t) {
   bh_ctx* c = t->ctx;
@@ -325,8 +314,13 @@ int bh_ctx_create(int device, bh_ctx** out) {
 
 void bh_ctx_destroy(bh_ctx* c) {
   if (!c) return;
+  for (bh_ctx*& L : c->lanes) {
+    bh_ctx_destroy(L);
+    L = nullptr;
+  }
   DeviceGuard g(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->stage) (void)hipHostFree(c->stage);
   {
     std::lock_guard<std::mutex> lk(c->mu);
     trim_locked(c);
@@ -395,6 +389,8 @@ int bh_ctx_profile_summary(bh_ctx* c, char* buf, int64_t cap) {
 
 int bh_ctx_trim(bh_ctx* c) {
   BH_REQ(c, "ctx is NULL");
+  for (bh_ctx* L : c->lanes)
+    if (L) BH_TRY(bh_ctx_trim(L));
   DeviceGuard g(c->device);
   BH_HIP(hipStreamSynchronize(c->stream));
   std::lock_guard<std::mutex> lk(c->mu);
@@ -542,8 +538,8 @@ int bh_tree_shape(const bh_tree* t, int64_t* n, int64_t* num_nodes, int32_t* dim
 
 int bh_tree_f32_stack(const bh_tree* t, int32_t* out) {
   BH_REQ(t && out, "tree and out must not be NULL");
-  out[0] = t->stack_hint[0];
-  out[1] = t->stack_hint[1];
+  out[0] = t->stack_hint[0].load();
+  out[1] = t->stack_hint[1].load();
   return BH_OK;
 }
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cstddef>
 #include <mutex>
 #include <map>
@@ -127,9 +128,27 @@ struct bh_ctx {
   };
   std::vector<ProfRec> prof_recs;
   std::vector<hipEvent_t> prof_pool;
+  // streamed targets: two lane contexts on the same device (own stream, own
+  // cache), created on first use; a lane's pinned host staging
+  bh_ctx* lanes[2] = {nullptr, nullptr};
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
 };
 
 namespace bh {
+// Makes `dev` the calling thread's current device for the scope.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
 // Records HIP events around the launches in its scope when profiling is on.
 struct ProfScope {
   bh_ctx* c;
@@ -201,7 +220,8 @@ struct bh_tree {
   // largest group-walk stack an earlier call on this tree had to grow to,
   // for self (0) and external (1) targets; later calls start there (the
   // walk's lists do not depend on the stack size)
-  int stack_hint[2] = {0, 0};
+  // (atomic: the streamed target path walks one tree from two host threads)
+  std::atomic<int> stack_hint[2]{{0}, {0}};
 };
 
 namespace bh {
@@ -290,6 +310,9 @@ struct F32Plan {
 };
 int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, double eps,
              double G, const bh_f32_opts* o, double* out, int64_t* stats, const F32Plan* plan = nullptr);
+// bh_sort.hip: the Morton order of external targets computed on the host,
+// bit for bit the order order_targets gives them on the device
+void host_target_order(const double* tgt, int64_t m, int dim, const double* box, uint32_t* order);
 // bh_api.hip: a new Morton order, particle order and monopoles for a tree
 // whose topology (a function of the system sizes) is kept
 int rebuild_in_place(bh_ctx* c, bh_tree* t, const double* pos, const double* mass);

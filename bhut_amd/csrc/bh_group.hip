@@ -1160,8 +1160,14 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
     set_error("multipole must be 1 (monopole) or 2 (quadrupole), got %d", multipole);
     return BH_EINVAL;
   }
-  int& hint = t->stack_hint[tgt ? 1 : 0];
-  if (!plan && hint > stack_cap) stack_cap = hint;
+  std::atomic<int>& hint = t->stack_hint[tgt ? 1 : 0];
+  if (!plan && hint.load() > stack_cap) stack_cap = hint.load();
+  // a grown stack is kept for later walks on the tree (never shrinks)
+  auto keep_hint = [&hint](int cap) {
+    int cur = hint.load();
+    while (cap > cur && !hint.compare_exchange_weak(cur, cap)) {
+    }
+  };
   if (!t->f32_ready) {
     ProfScope p(c, "f32_prepare");
     int rc = f32_prepare(c, t);
@@ -1423,7 +1429,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
         return BH_EOVERFLOW;
       }
       stack_cap = std::min(MAX_STACK, 2 * stack_cap);
-      hint = std::max(hint, stack_cap);
+      keep_hint(stack_cap);
       BH_TRY(count_pass());
     }
   }
@@ -1452,7 +1458,7 @@ int walk_f32(bh_ctx* c, bh_tree* t, const double* tgt, int64_t m, double theta, 
         return BH_EOVERFLOW;
       }
       stack_cap = std::min(MAX_STACK, 2 * stack_cap);
-      hint = std::max(hint, stack_cap);
+      keep_hint(stack_cap);
       int rc = fill_u32(c, err, 0u, 1);
       if (!rc) rc = count_pass();
       if (rc) return rc;

@@ -11,6 +11,8 @@
 // keys in input order -- the stability np.argsort(kind='stable') guarantees.
 #include "bh_common.h"
 
+#include <algorithm>
+
 namespace bh {
 
 // ---------------------------------------------------------------- bbox
@@ -114,7 +116,7 @@ int bbox(bh_ctx* c, const double* pos, int64_t n, int dim, double* d_box) {
 // ---------------------------------------------------------------- Morton
 // Bit spreading for 21-bit coordinates: the standard shift-and-mask
 // interleave (each step halves the run length of consecutive bits).
-__device__ __forceinline__ uint64_t spread3(uint64_t x) {
+__host__ __device__ __forceinline__ uint64_t spread3(uint64_t x) {
   x &= 0x1fffffull;
   x = (x | (x << 32)) & 0x1f00000000ffffull;
   x = (x | (x << 16)) & 0x1f0000ff0000ffull;
@@ -123,7 +125,7 @@ __device__ __forceinline__ uint64_t spread3(uint64_t x) {
   x = (x | (x << 2)) & 0x1249249249249249ull;
   return x;
 }
-__device__ __forceinline__ uint64_t spread2(uint64_t x) {
+__host__ __device__ __forceinline__ uint64_t spread2(uint64_t x) {
   x &= 0x1fffffull;
   x = (x | (x << 16)) & 0x0000ffff0000ffffull;
   x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
@@ -138,9 +140,11 @@ __device__ __forceinline__ uint64_t spread2(uint64_t x) {
 // conversion NumPy uses (2^63) becomes after the clamp.  clamp=1 is for
 // external targets, which may lie outside the box: they are clamped into it
 // (their codes only order the targets for coherent waves).
+// Also compiled for the host (host_target_order): the same IEEE operations,
+// with floating-point contraction off in both.
 template <int DIM>
-__device__ __forceinline__ uint64_t morton_code(const double* __restrict__ p, const double* __restrict__ box,
-                                                int clamp) {
+__host__ __device__ __forceinline__ uint64_t morton_code(const double* __restrict__ p,
+                                                         const double* __restrict__ box, int clamp) {
   const double cap = 1.0 - 1e-12;
   uint64_t code = 0;
 #pragma unroll
@@ -286,6 +290,16 @@ int morton_iota(bh_ctx* c, const double* pos, int64_t n, int dim, const double* 
     hipLaunchKernelGGL(k_morton<2>, dim3(nb), dim3(256), 0, c->stream, pos, n, d_box, clamp, keys, vals);
   BH_LAUNCH_CHECK();
   return BH_OK;
+}
+
+void host_target_order(const double* tgt, int64_t m, int dim, const double* box, uint32_t* order) {
+  std::vector<uint64_t> key(m);
+  for (int64_t i = 0; i < m; i++) {
+    key[i] = dim == 3 ? morton_code<3>(tgt + i * 3, box, 1) : morton_code<2>(tgt + i * 2, box, 1);
+    order[i] = (uint32_t)i;
+  }
+  // stable: ties keep input order, as the device's LSD radix sort does
+  std::stable_sort(order, order + m, [&key](uint32_t a, uint32_t b) { return key[a] < key[b]; });
 }
 
 // ---------------------------------------------------------------- radix sort

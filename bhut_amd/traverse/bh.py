@@ -59,19 +59,28 @@ def barnes_hut_accelerations(tree: TreeData, positions, masses, softening, theta
 
 
 def barnes_hut_accelerations_targets(tree: TreeData, tree_positions, tree_masses, target_positions, softening,
-                                     theta: float = 0.5, G: float = 1.0, *, precision: str = "fp64") -> np.ndarray:
-    """Accelerations at external targets, no self exclusion (bh.py:30-80)."""
+                                     theta: float = 0.5, G: float = 1.0, *, precision: str = "fp64",
+                                     chunk_size: int | None = None) -> np.ndarray:
+    """Accelerations at external targets, no self exclusion (bh.py:30-80).
+    With chunk_size the targets stream through the device in chunks on two
+    streams (the Dask target chunks of api.py:28-78 and dask_.py:173-228),
+    bit-identical to the unchunked call; device memory then holds two chunks
+    of targets whatever their number."""
     validate_kernel_inputs(tree_positions, tree_masses, softening)
     eps = _scalar_softening(softening)
     tgt = np.asarray(target_positions, dtype=np.float64)
     if tgt.ndim != 2 or tgt.shape[1] != tree_positions.shape[1]:
         raise ValueError(f"target_positions must have shape (M, {tree_positions.shape[1]}), got {tgt.shape}")
+    if precision not in ("fp64", "fp32"):
+        raise ValueError(f"precision must be 'fp64' or 'fp32', got '{precision}'")
     dt = device_tree_for(tree, tree_positions, tree_masses)
+    if chunk_size is not None:
+        if int(chunk_size) <= 0:
+            raise ValueError(f"chunk_size must be positive, got {chunk_size}")
+        return dt.accel_targets_streamed(tgt, theta, eps, G, precision=precision, chunk=int(chunk_size))
     if precision == "fp64":
         return dt.accel_targets(tgt, theta, eps, G)
-    if precision == "fp32":
-        return dt.accel_f32(tgt, theta, eps, G)
-    raise ValueError(f"precision must be 'fp64' or 'fp32', got '{precision}'")
+    return dt.accel_f32(tgt, theta, eps, G)
 
 
 def interaction_lists(tree: TreeData, positions, masses, theta: float = 0.5, targets=None):

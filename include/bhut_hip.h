@@ -150,6 +150,21 @@ int bh_accel_targets_dev_noalloc(bh_tree *t, const double *d_targets, int64_t m,
                                  double softening, double G, double *d_out);
 
 /* ------------------------------------------- accelerations (FP32, chosen) */
+/* Accelerations at m HOST-resident targets streamed through the device in
+ * chunks (the Dask target chunks of bhut/api.py:28-78 and api.py:690-731,
+ * map_blocks in bhut/backends/dask_.py:173-228): two lane streams, each
+ * driven by a host thread of its own, take alternate chunks through pinned
+ * staging memory (gather, copy in, walk, copy out, scatter), so one lane's
+ * copies and host work overlap the other's kernels; device memory holds two
+ * chunks whatever m is.  precision 64: the plain walk (bh_accel_targets);
+ * 32: the FP32 form (bh_accel_f32_targets, opts as there).  chunk <= 0
+ * selects 1048576; chunks are rounded up to whole groups of 64.  The targets
+ * are put in the tree's Morton order on the host first, so every result is
+ * bit for bit what one call on all targets returns.  Synchronous. */
+int bh_accel_targets_streamed(bh_tree *t, const double *targets, int64_t m, int64_t chunk, double theta,
+                              double softening, double G, int32_t precision, const bh_f32_opts *opts,
+                              double *out);
+
 /* Same contract as bh_accel_self / bh_accel_targets, computed in FP32 on
  * coordinates relative to leaf and group centres, with a conservative group
  * opening test; opts may be NULL for defaults.  stats (may be NULL) receives
