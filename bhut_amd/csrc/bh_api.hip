@@ -191,6 +191,7 @@ static int build_dev(bh_ctx* c, const double* pos, const double* mass, const int
   t->ctx = c;
   t->dim = dim;
   t->leaf = leaf;
+  t->max_leaf = leaf;  // a split node has more than `leaf` particles, a leaf at most `leaf`
   int64_t n = 0;
   for (int64_t s = 0; s < nsys; s++) n += counts[s];
   t->n = n;
@@ -644,6 +645,8 @@ int bh_tree_import(bh_ctx* c, int64_t n, int64_t M, int32_t dim, int32_t leaf_si
   t->n = n;
   t->dim = dim;
   t->leaf = leaf_size;
+  t->max_leaf = 0;
+  for (int32_t v : leaves) t->max_leaf = std::max<int64_t>(t->max_leaf, count[v]);
   t->M = M;
   t->nleaves = (int64_t)leaves.size();
   t->depth = 0;
@@ -717,6 +720,7 @@ int bh_tree_clone(const bh_tree* src, bh_tree** out) {
   t->n = src->n;
   t->dim = src->dim;
   t->leaf = src->leaf;
+  t->max_leaf = src->max_leaf;
   t->M = src->M;
   t->nleaves = src->nleaves;
   t->depth = src->depth;

@@ -350,8 +350,11 @@ __device__ __forceinline__ bool arrive(uint32_t* flags, int node) {
 }
 
 // Leaf v: tight box (build.py:232-242), mass and centre of mass in NumPy
-// order (build.py:258-272); REFIT keeps the box and centre.
-template <int DIM, bool REFIT>
+// order (build.py:258-272); REFIT keeps the box and centre.  BIG: the tree
+// has leaves of more than 128 particles, whose masses take NumPy's
+// recursive pairwise split (its explicit stack is per-lane private memory,
+// so trees without such leaves use kernels that do not carry it).
+template <int DIM, bool REFIT, bool BIG>
 __device__ __forceinline__ void up_leaf(int v, const int32_t* __restrict__ start, const int32_t* __restrict__ count,
                                         const double4* __restrict__ spart, double* center, double* half,
                                         double* mass, double* com, double* bmin, double* bmax) {
@@ -384,7 +387,7 @@ __device__ __forceinline__ void up_leaf(int v, const int32_t* __restrict__ start
 #pragma unroll
     for (int d = 0; d < DIM; d++) ctr[d] = center[v * DIM + d];
   }
-  const double total = 0.0 + np_pairwise_mass(p, c);
+  const double total = 0.0 + (BIG ? np_pairwise_mass(p, c) : pw_block(p, c));
   mass[v] = total;
   if (total > 0) {
     double sx[DIM];
@@ -474,7 +477,7 @@ __device__ __forceinline__ void up_node(int node, const int32_t* __restrict__ fi
 // Trees without depth tables (imported): one thread per leaf, leaves in
 // left-to-right (memory) order; after its leaf a thread climbs, and the
 // second thread to reach a parent combines its two children.
-template <int DIM, bool REFIT>
+template <int DIM, bool REFIT, bool BIG>
 __global__ __launch_bounds__(256) void k_leaf_up(
     int64_t nleaves, const int32_t* __restrict__ leaf_order, const int32_t* __restrict__ start,
     const int32_t* __restrict__ count, const int32_t* __restrict__ first_child,
@@ -483,7 +486,7 @@ __global__ __launch_bounds__(256) void k_leaf_up(
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nleaves) return;
   const int v = leaf_order[t];
-  up_leaf<DIM, REFIT>(v, start, count, spart, center, half, mass, com, bmin, bmax);
+  up_leaf<DIM, REFIT, BIG>(v, start, count, spart, center, half, mass, com, bmin, bmax);
   int node = parent[v];
   while (node >= 0) {
     if (!arrive(flags, node)) return;
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(256) void k_leaf_up(
 
 // Trees built here: one launch per depth, deepest first, so a node's
 // children are complete when it is combined (no atomics, no fences).
-template <int DIM, bool REFIT>
+template <int DIM, bool REFIT, bool BIG>
 __global__ __launch_bounds__(256) void k_up_level(
     int d, int64_t nthreads, const int32_t* __restrict__ lvl_sys, const int64_t* __restrict__ slot_off,
     const int32_t* __restrict__ slot_node, const int32_t* __restrict__ start, const int32_t* __restrict__ count,
@@ -504,7 +507,7 @@ __global__ __launch_bounds__(256) void k_up_level(
   if (t >= nthreads) return;
   const int v = level_node(d, t, lvl_sys, slot_off, slot_node);
   if (v < 0) return;
-  if (first_child[v] < 0) up_leaf<DIM, REFIT>(v, start, count, spart, center, half, mass, com, bmin, bmax);
+  if (first_child[v] < 0) up_leaf<DIM, REFIT, BIG>(v, start, count, spart, center, half, mass, com, bmin, bmax);
   else up_node<DIM, REFIT>(v, first_child, center, half, mass, com, bmin, bmax);
 }
 
@@ -521,19 +524,28 @@ int fill_u32(bh_ctx* c, uint32_t* p, uint32_t v, int64_t n) {
 }
 
 int upward(bh_ctx* c, bh_tree* t, int refit) {
+  const bool big = t->max_leaf > 128;
   if (t->slot_node) {
     for (int d = t->dmax; d >= 0; d--) {
       const int64_t nthr = t->lvl_nsys[d] << d;
       const int32_t* ls = t->d_lvl_sys ? t->d_lvl_sys + t->h_lvl_off[d] : nullptr;
       const unsigned nb = (unsigned)cdiv(nthr, 256);
-#define BH_UPL(D, R)                                                                                        \
-  hipLaunchKernelGGL((k_up_level<D, R>), dim3(nb), dim3(256), 0, c->stream, d, nthr, ls, t->d_slot_off,     \
+#define BH_UPL(D, R, B)                                                                                     \
+  hipLaunchKernelGGL((k_up_level<D, R, B>), dim3(nb), dim3(256), 0, c->stream, d, nthr, ls, t->d_slot_off,  \
                      t->slot_node, t->start, t->count, t->first_child, t->spart, t->center, t->half, t->mass, \
                      t->com, t->bmin, t->bmax)
-      if (t->dim == 3) {
-        if (refit) BH_UPL(3, true); else BH_UPL(3, false);
+      if (big) {
+        if (t->dim == 3) {
+          if (refit) BH_UPL(3, true, true); else BH_UPL(3, false, true);
+        } else {
+          if (refit) BH_UPL(2, true, true); else BH_UPL(2, false, true);
+        }
       } else {
-        if (refit) BH_UPL(2, true); else BH_UPL(2, false);
+        if (t->dim == 3) {
+          if (refit) BH_UPL(3, true, false); else BH_UPL(3, false, false);
+        } else {
+          if (refit) BH_UPL(2, true, false); else BH_UPL(2, false, false);
+        }
       }
 #undef BH_UPL
       BH_LAUNCH_CHECK();
@@ -546,14 +558,22 @@ int upward(bh_ctx* c, bh_tree* t, int refit) {
   int rc = fill_u32(c, flags, 0u, t->M);
   if (rc) return rc;
   const unsigned nb = (unsigned)cdiv(t->nleaves, 256);
-#define BH_UP(D, R)                                                                            \
-  hipLaunchKernelGGL((k_leaf_up<D, R>), dim3(nb), dim3(256), 0, c->stream, t->nleaves,          \
+#define BH_UP(D, R, B)                                                                         \
+  hipLaunchKernelGGL((k_leaf_up<D, R, B>), dim3(nb), dim3(256), 0, c->stream, t->nleaves,       \
                      t->leaf_order, t->start, t->count, t->first_child, t->parent, t->spart,    \
                      t->center, t->half, t->mass, t->com, t->bmin, t->bmax, flags)
-  if (t->dim == 3) {
-    if (refit) BH_UP(3, true); else BH_UP(3, false);
+  if (big) {
+    if (t->dim == 3) {
+      if (refit) BH_UP(3, true, true); else BH_UP(3, false, true);
+    } else {
+      if (refit) BH_UP(2, true, true); else BH_UP(2, false, true);
+    }
   } else {
-    if (refit) BH_UP(2, true); else BH_UP(2, false);
+    if (t->dim == 3) {
+      if (refit) BH_UP(3, true, false); else BH_UP(3, false, false);
+    } else {
+      if (refit) BH_UP(2, true, false); else BH_UP(2, false, false);
+    }
   }
 #undef BH_UP
   BH_LAUNCH_CHECK();

@@ -163,6 +163,7 @@ struct bh_tree {
   int64_t n = 0;
   int dim = 3;
   int leaf = 32;
+  int64_t max_leaf = 32;  // particles in the largest leaf (at most `leaf` for built trees)
   int64_t M = 0;
   int64_t nleaves = 0;
   int depth = 0;
