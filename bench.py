@@ -17,6 +17,8 @@ JSON line; the default run prints only the headline line):
              leaf / group size / stack depth autotuned per theta
   leapfrog   4M Plummer, 100 KDK steps, refit every step, rebuild every 10,
              steps captured in HIP graphs; FP32 vs FP64 forces
+  stream     16M host-resident targets against a 1M Plummer tree, streamed in
+             chunks on two streams vs one call vs the device-resident walk
   copy       three 256 MiB device-to-device copies: the known-size kernel that
              calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters
 """
@@ -53,7 +55,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=["plummer", "ensemble", "diskhalo", "leapfrog", "copy"], default="plummer")
+    p.add_argument("--workload", choices=["plummer", "ensemble", "diskhalo", "leapfrog", "stream", "copy"],
+                   default="plummer")
     p.add_argument("--n", type=int, default=0, help="particles (0: the workload's own size)")
     p.add_argument("--theta", type=float, default=0.5)
     p.add_argument("--softening", type=float, default=0.01)
@@ -619,6 +622,66 @@ def workload_leapfrog(args, env):
     return line
 
 
+# ---------------------------------------------------------------- streamed targets
+def workload_stream(args, env):
+    """1M Plummer tree; 16M host-resident targets uniform in [-5, 5]^3
+    through the device in chunks on two streams, against one call on all
+    targets (host arrays) and against the device-resident walk alone."""
+    from bhut_amd import workloads
+    nat, torch = env.nat, env.torch
+    N = 1_000_000
+    M = args.n or 16_000_000
+    theta, eps, G, leaf = args.theta, args.softening, 1.0, args.leaf
+    x, m = workloads.plummer(N, seed=1)
+    rng = np.random.default_rng(12)
+    q = rng.uniform(-5.0, 5.0, (M, 3))
+    t = nat.DeviceTree.build(x, m, leaf, context=env.ctx)
+    q_t = env.upload(q)
+    o_t = torch.empty((M, 3), dtype=torch.float64, device=env.dev)
+    env.sync()
+    opts = nat.F32Opts(64, 1, 1024, 2)
+    res = {}
+    for form in ("fp32", "fp64"):
+        reps = 3 if form == "fp32" else 1
+        def dev_call():
+            if form == "fp32":
+                t.accel_f32_targets_dev(q_t.data_ptr(), M, o_t.data_ptr(), theta, eps, G, opts)
+            else:
+                t.accel_targets_dev(q_t.data_ptr(), M, o_t.data_ptr(), theta, eps, G)
+        dms, _, _ = env.timed(dev_call, reps, 1)
+        one = t.accel_f32(q, theta, eps, G, opts) if form == "fp32" else t.accel_targets(q, theta, eps, G)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            one = t.accel_f32(q, theta, eps, G, opts) if form == "fp32" else t.accel_targets(q, theta, eps, G)
+        one_ms = (time.perf_counter() - t0) * 1e3 / reps
+        row = {"device_resident_ms": dms, "one_call_host_ms": one_ms, "chunks": {}}
+        for chunk in (1 << 20, 1 << 22):
+            got = t.accel_targets_streamed(q, theta, eps, G, precision=form, chunk=chunk, opts=opts)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                got = t.accel_targets_streamed(q, theta, eps, G, precision=form, chunk=chunk, opts=opts)
+            sms = (time.perf_counter() - t0) * 1e3 / reps
+            same = bool(np.array_equal(got.view(np.uint64), one.view(np.uint64)))
+            row["chunks"][str(chunk)] = {"ms": sms, "targets_per_s": M / (sms * 1e-3),
+                                         "bit_identical_to_one_call": same,
+                                         "device_bytes_of_targets": 2 * chunk * 48}
+            log(f"bench: stream {form} chunk {chunk}: {sms:.1f} ms (one call {one_ms:.1f} ms, device-resident "
+                f"{dms:.1f} ms); identical {same}")
+        res[form] = row
+    head = res["fp32"]["chunks"][str(1 << 20)]
+    line = base_line(args, M / (head["ms"] * 1e-3), head["ms"], "fp32",
+                     {"workload": "streamed_targets", "n_sources": N, "targets": M,
+                      "targets_distribution": "uniform [-5,5]^3, seed 12", "theta": theta, "softening": eps,
+                      "leaf_size": leaf, "G": G, "chunk": 1 << 20,
+                      "step": "host targets -> device in chunks on two streams -> host accelerations"})
+    line["metric"] = "target-accels/s, 16M host targets streamed against a 1M Plummer tree"
+    line["unit"] = "target-accels/s"
+    line["forms"] = res
+    line["roofline"] = None
+    line["cpu_baseline"] = None
+    return line
+
+
 def workload_copy(args, env):
     torch = env.torch
     nbytes = 256 << 20
@@ -645,7 +708,7 @@ def main():
     if args.list_budget_gib > 0:
         env.ctx.set_list_budget(int(args.list_budget_gib * 2 ** 30))
     fn = {"plummer": workload_plummer, "ensemble": workload_ensemble, "diskhalo": workload_diskhalo,
-          "leapfrog": workload_leapfrog, "copy": workload_copy}[args.workload]
+          "leapfrog": workload_leapfrog, "stream": workload_stream, "copy": workload_copy}[args.workload]
     line = fn(args, env)
     print(json.dumps(line), flush=True)
 

@@ -12,6 +12,7 @@
 #include "bh_common.h"
 
 #include <algorithm>
+#include <thread>
 
 namespace bh {
 
@@ -292,14 +293,65 @@ int morton_iota(bh_ctx* c, const double* pos, int64_t n, int dim, const double* 
   return BH_OK;
 }
 
+// Stable LSD radix sort of the targets' codes on the host threads: 8-bit
+// digits; each thread counts and scatters its own contiguous block, and the
+// blocks' offsets within a digit follow block order, so equal codes keep
+// input order (as the device sort keeps them).
 void host_target_order(const double* tgt, int64_t m, int dim, const double* box, uint32_t* order) {
-  std::vector<uint64_t> key(m);
-  for (int64_t i = 0; i < m; i++) {
-    key[i] = dim == 3 ? morton_code<3>(tgt + i * 3, box, 1) : morton_code<2>(tgt + i * 2, box, 1);
-    order[i] = (uint32_t)i;
+  const int nthr = (int)std::max<int64_t>(1, std::min<int64_t>(
+      std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())), m / 65536 + 1));
+  std::vector<uint64_t> k0(m), k1(m);
+  std::vector<uint32_t> v1(m);
+  auto blk = [&](int t) { return std::make_pair(m * t / nthr, m * (t + 1) / nthr); };
+  auto run = [&](auto&& fn) {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nthr; t++) th.emplace_back(fn, t);
+    fn(0);
+    for (auto& x : th) x.join();
+  };
+  uint64_t kor = 0;
+  std::vector<uint64_t> kor_t(nthr, 0);
+  run([&](int t) {
+    const auto [a, b] = blk(t);
+    uint64_t o = 0;
+    for (int64_t i = a; i < b; i++) {
+      k0[i] = dim == 3 ? morton_code<3>(tgt + i * 3, box, 1) : morton_code<2>(tgt + i * 2, box, 1);
+      order[i] = (uint32_t)i;
+      o |= k0[i];
+    }
+    kor_t[t] = o;
+  });
+  for (uint64_t o : kor_t) kor |= o;
+  uint64_t *ka = k0.data(), *kb = k1.data();
+  uint32_t *va = order, *vb = v1.data();
+  std::vector<int64_t> hist((size_t)nthr * 256);
+  for (int shift = 0; shift < 64 && (kor >> shift) != 0; shift += 8) {
+    run([&](int t) {
+      const auto [a, b] = blk(t);
+      int64_t* h = hist.data() + (size_t)t * 256;
+      std::fill(h, h + 256, 0);
+      for (int64_t i = a; i < b; i++) h[(ka[i] >> shift) & 255]++;
+    });
+    int64_t acc = 0;  // exclusive offsets: digit-major, then block order
+    for (int d = 0; d < 256; d++)
+      for (int t = 0; t < nthr; t++) {
+        const int64_t c = hist[(size_t)t * 256 + d];
+        hist[(size_t)t * 256 + d] = acc;
+        acc += c;
+      }
+    run([&](int t) {
+      const auto [a, b] = blk(t);
+      int64_t* h = hist.data() + (size_t)t * 256;
+      for (int64_t i = a; i < b; i++) {
+        const int64_t p = h[(ka[i] >> shift) & 255]++;
+        kb[p] = ka[i];
+        vb[p] = va[i];
+      }
+    });
+    std::swap(ka, kb);
+    std::swap(va, vb);
   }
-  // stable: ties keep input order, as the device's LSD radix sort does
-  std::stable_sort(order, order + m, [&key](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+  if (va != order) std::copy(va, va + m, order);
 }
 
 // ---------------------------------------------------------------- radix sort
