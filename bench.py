@@ -358,6 +358,10 @@ def workload_plummer(args, env):
             tf = ceil["uniform"] * FLOPS_PER_INTERACTION / 1e12
             roof["loop_ceiling"] = {"pairs_per_s": ceil["uniform"], "tflops": tf, "frac": roof["achieved"] / tf,
                                     "mass_pairs_per_s": ceil["mass"], "no_rsq_pairs_per_s": ceil["no_rsq"]}
+            if "fma_lane_per_s" in ceil:
+                # independent FP32 FMAs alone (2 flops each): the vector ALU's
+                # issue ceiling on this device without packed instructions
+                roof["valu_fma_ceiling_tflops"] = 2 * ceil["fma_lane_per_s"] / 1e12
             print(f"bench: pair-loop ceiling {tf:.1f} TFLOP/s; evaluation at {roof['achieved'] / tf:.1%} of it",
                   file=sys.stderr)
     line["roofline"] = roof
