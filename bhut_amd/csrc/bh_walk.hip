@@ -29,8 +29,39 @@
 // at once, where the per-lane walk of the same lists ran only the lanes that
 // happened to be at a leaf.
 #include "bh_common.h"
+#include "bh_fp64.h"
 
 namespace bh {
+
+// The Numba leaf formula (bh.py:249-278) for one leaf with the middle-range
+// square root and reciprocal: r^2 + eps^2 >= eps^2 >= 2^-700 holds by the
+// caller's choice of this path, so only the upper end is checked; returns
+// whether some pair of this lane left the range (the caller then recomputes
+// the leaf with sqrt() and '/').  G1: G == 1, where G * m == m exactly.
+template <int DIM, bool SELF, bool G1>
+__device__ __forceinline__ bool leaf_mid(const double4* __restrict__ spart, int j0, int j1, int64_t t, bool mine,
+                                         double tx, double ty, double tz, double eps2, double G, double& px,
+                                         double& py, double& pz) {
+  bool bad = false;
+  for (int j = j0; j < j1; j++) {
+    const double4 q = spart[j];
+    if (mine && !(SELF && j == t)) {
+      const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+      double rsq = dx * dx;
+      rsq = rsq + dy * dy;
+      if (DIM == 3) rsq = rsq + dz * dz;
+      rsq = rsq + eps2;
+      bad |= !(rsq <= 0x1p700);
+      const double rinv = recip_mid(sqrt_mid(rsq));
+      const double ric = rinv * rinv * rinv;
+      const double mag = (G1 ? q.w : G * q.w) * ric;
+      px = px + mag * dx;
+      py = py + mag * dy;
+      if (DIM == 3) pz = pz + mag * dz;
+    }
+  }
+  return bad;
+}
 
 template <int DIM, bool SELF, int MODE>
 __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
@@ -68,6 +99,10 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
   int64_t nl = 0, wpos = 0;
   int64_t c_pop = 0, c_test = 0, c_mono = 0, c_leaf = 0, c_pair = 0;
   if (MODE == WALK_LFILL && live) wpos = lptr[orig];
+  // eps^2 in [2^-700, 2^690]: the leaf pairs take the middle-range square
+  // root and reciprocal (every r^2 + eps^2 is at least eps^2)
+  const bool fast_eps = eps2 >= 0x1p-700 && eps2 <= 0x1p690;
+  const bool g_one = G == 1.0;
   // a forest walks each system alone: the lanes of one system at a time
   const int64_t sys = (SELF && nsys > 1 && live) ? find_sys(sys_poff, nsys, t) : 0;
   uint64_t todo = __ballot(live);
@@ -102,25 +137,36 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
         const int j0 = nd.start, j1 = nd.start + nd.count;
         if (MODE == WALK_ACC) {
           double px = 0.0, py = 0.0, pz = 0.0;
+          bool redo = true;
+          if (fast_eps) {
+            const bool bad = g_one ? leaf_mid<DIM, SELF, true>(spart, j0, j1, t, mine, tx, ty, tz, eps2, G, px,
+                                                               py, pz)
+                                   : leaf_mid<DIM, SELF, false>(spart, j0, j1, t, mine, tx, ty, tz, eps2, G, px,
+                                                                py, pz);
+            redo = __ballot(bad) != 0;
+          }
           bool zdiv = false;
           // Numba loop (bh.py:249-278): r = sqrt(r2 + eps^2); 1/r; r^-3.
           // The sources are the same for every lane (scalar loads).
-          for (int j = j0; j < j1; j++) {
-            const double4 q = spart[j];
-            if (mine && !(SELF && j == t)) {
-              const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
-              double rsq = dx * dx;
-              rsq = rsq + dy * dy;
-              if (DIM == 3) rsq = rsq + dz * dz;
-              rsq = rsq + eps2;
-              const double r = sqrt(rsq);
-              zdiv |= r == 0.0;  // Numba raises ZeroDivisionError here
-              const double rinv = 1.0 / r;
-              const double ric = rinv * rinv * rinv;
-              const double mag = G * q.w * ric;
-              px = px + mag * dx;
-              py = py + mag * dy;
-              if (DIM == 3) pz = pz + mag * dz;
+          if (redo) {
+            px = py = pz = 0.0;
+            for (int j = j0; j < j1; j++) {
+              const double4 q = spart[j];
+              if (mine && !(SELF && j == t)) {
+                const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+                double rsq = dx * dx;
+                rsq = rsq + dy * dy;
+                if (DIM == 3) rsq = rsq + dz * dz;
+                rsq = rsq + eps2;
+                const double r = sqrt(rsq);
+                zdiv |= r == 0.0;  // Numba raises ZeroDivisionError here
+                const double rinv = 1.0 / r;
+                const double ric = rinv * rinv * rinv;
+                const double mag = G * q.w * ric;
+                px = px + mag * dx;
+                py = py + mag * dy;
+                if (DIM == 3) pz = pz + mag * dz;
+              }
             }
           }
           if (zdiv) {
@@ -173,18 +219,27 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
         v = nd.skip;
         continue;
       }
-      bool open = false;
+      // opening test (bh.py:586-601): d = ||t - com|| as OpenBLAS ddot forms
+      // it (x*x, then fma for y and z), open when d == 0 or size/d >= theta;
+      // the middle-range square root and division when the node's size and
+      // every visiting lane's d^2 allow them (then d > 0)
+      double d2 = 0.0;
       if (mine) {
-        // opening test (bh.py:586-601): d = ||t - com|| as OpenBLAS ddot
-        // forms it (x*x, then fma for y and z), open when d == 0 or
-        // size/d >= theta
-        if (MODE == WALK_STATS) c_test++;
         const double ux = tx - nd.cx, uy = ty - nd.cy, uz = tz - nd.cz;
-        double d2 = ux * ux;
+        d2 = ux * ux;
         d2 = fma(uy, uy, d2);
         if (DIM == 3) d2 = fma(uz, uz, d2);
+      }
+      const bool mid = nd.size >= 0x1p-300 && nd.size <= 0x1p300 && d2 >= 0x1p-700 && d2 <= 0x1p700;
+      bool open = false;
+      if (__ballot(mine && !mid) == 0) {
+        if (mine) open = div_mid(nd.size, sqrt_mid(d2)) >= theta;
+      } else if (mine) {
         const double dist = sqrt(d2);
         open = (dist == 0.0) || (nd.size / dist >= theta);
+      }
+      if (mine) {
+        if (MODE == WALK_STATS) c_test++;
         if (open) {
           follow = dv + 1;
         } else {

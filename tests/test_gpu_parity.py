@@ -405,3 +405,26 @@ def test_direct_sum_matches_oracle(native, dim):
     # theta = 0 walk == direct sum (bh.py:600-601)
     dt = native.DeviceTree.build(x, m, 32)
     acc_close(dt.accel_self(0.0, 0.02, 1.0), O.direct(x, m, softening=0.02))
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_theta0_walk_bitwise_against_oracle(native, dim):
+    """At theta = 0 every node is opened and every leaf summed directly
+    (bh.py:600-601), so nothing but the leaf formula and the pop order enter
+    the result, and the plain walk must equal the oracle (glibc's correctly
+    rounded sqrt, IEEE division, the reference's summation order) bit for
+    bit.  Softened (the walk's middle-range square root and reciprocal), at
+    G = 1 (the mass product skipped, exact) and at other G, and unsoftened
+    (sqrt() and '/'); self and external targets."""
+    rng = np.random.default_rng(70 + dim)
+    n = 6000
+    x = rng.standard_normal((n, dim))
+    m = rng.uniform(0.5, 1.5, n)
+    q = rng.uniform(-3, 3, (1500, dim))
+    dt = native.DeviceTree.build(x, m, 16)
+    ot = O.build(x, m, 16)
+    for eps, G in ((0.01, 1.0), (0.01, 6.674e-11), (1e-3, 2.5), (0.0, 1.0)):
+        bits_equal(dt.accel_self(0.0, eps, G), O.accelerations(ot, x, m, softening=eps, theta=0.0, G=G),
+                   f"self eps {eps} G {G}")
+        bits_equal(dt.accel_targets(q, 0.0, eps, G), O.accelerations(ot, x, m, q, softening=eps, theta=0.0, G=G),
+                   f"targets eps {eps} G {G}")
