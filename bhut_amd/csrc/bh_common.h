@@ -328,6 +328,15 @@ __device__ __forceinline__ double np_max(double a, double b) { return (a > b || 
 
 // System owning particle i of a forest: the last s with poff[s] <= i.
 // Systems are consecutive runs both in input order and in sorted order.
+// Each wavefront touches only LDS it owns (its stack, its tile), and a
+// wavefront's LDS operations complete in issue order, so a compiler-level
+// wave barrier orders its own writes and reads; no workgroup barrier.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 __device__ __forceinline__ int64_t find_sys(const int32_t* __restrict__ poff, int64_t nsys, int64_t i) {
   int64_t lo = 0, hi = nsys - 1;
   while (lo < hi) {

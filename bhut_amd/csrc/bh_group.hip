@@ -270,14 +270,6 @@ __global__ void k_gather_targets(const double* __restrict__ tgt, const uint32_t*
 }
 
 // ---------------------------------------------------------------- walk
-// Each wavefront touches only LDS it owns (its stack, its tile), and a
-// wavefront's LDS operations complete in issue order, so a compiler-level
-// wave barrier orders its own writes and reads; no workgroup barrier.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-}
 
 constexpr int MAX_STACK = 8192;  // LDS stack entries per wavefront, at most
 constexpr int WALK_WAVES = 4;    // groups (wavefronts) per workgroup, each with its own LDS stack

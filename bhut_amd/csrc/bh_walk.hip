@@ -38,13 +38,18 @@ namespace bh {
 // caller's choice of this path, so only the upper end is checked; returns
 // whether some pair of this lane left the range (the caller then recomputes
 // the leaf with sqrt() and '/').  G1: G == 1, where G * m == m exactly.
+// The leaf's sources pass through the wavefront's LDS tile `wt`, 64 at a time
+// (one coalesced load per chunk instead of one scalar load, and its wait,
+// per source), and LEAF_READS of them are read from the tile before their
+// pairs are computed; every lane still adds its pairs in source order.
+constexpr int LEAF_READS = 4;
+
 template <int DIM, bool SELF, bool G1>
-__device__ __forceinline__ bool leaf_mid(const double4* __restrict__ spart, int j0, int j1, int64_t t, bool mine,
-                                         double tx, double ty, double tz, double eps2, double G, double& px,
-                                         double& py, double& pz) {
+__device__ __forceinline__ bool leaf_mid(double4* __restrict__ wt, const double4* __restrict__ spart, int j0,
+                                         int j1, int64_t t, bool mine, double tx, double ty, double tz, double eps2,
+                                         double G, double& px, double& py, double& pz) {
   bool bad = false;
-  for (int j = j0; j < j1; j++) {
-    const double4 q = spart[j];
+  auto pair = [&](const double4& q, int j) {
     if (mine && !(SELF && j == t)) {
       const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
       double rsq = dx * dx;
@@ -59,12 +64,31 @@ __device__ __forceinline__ bool leaf_mid(const double4* __restrict__ spart, int 
       py = py + mag * dy;
       if (DIM == 3) pz = pz + mag * dz;
     }
+  };
+  const int lane = threadIdx.x % WAVE;
+  for (int c0 = j0; c0 < j1; c0 += WAVE) {
+    const int cn = j1 - c0 < WAVE ? j1 - c0 : WAVE;
+    wave_sync();  // every read of the previous chunk precedes these writes
+    if (lane < cn) wt[lane] = spart[c0 + lane];
+    wave_sync();
+    int k = 0;
+    for (; k + LEAF_READS <= cn; k += LEAF_READS) {
+      double4 q[LEAF_READS];
+#pragma unroll
+      for (int u = 0; u < LEAF_READS; u++) q[u] = wt[k + u];
+#pragma unroll
+      for (int u = 0; u < LEAF_READS; u++) pair(q[u], c0 + k + u);
+    }
+    for (; k < cn; k++) pair(wt[k], c0 + k);
   }
   return bad;
 }
 
+// At most 74 SGPRs (80 with VCC and the reserved ones): a gfx950 SIMD holds
+// eight wavefronts of a kernel with 80 SGPRs but only seven with 82 and up
+// (measured), and the compiler's own choice for this kernel was 86.
 template <int DIM, bool SELF, int MODE>
-__global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(74))) void k_walk(const NodeD* __restrict__ nodes,
                                               const double4* __restrict__ spart,
                                               const int32_t* __restrict__ perm, int64_t m,
                                               const double* __restrict__ tgt,
@@ -75,6 +99,7 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
                                               int64_t* __restrict__ stats,
                                               const int32_t* __restrict__ sys_poff,
                                               const int32_t* __restrict__ sys_noff, int64_t nsys) {
+  __shared__ double4 ltile[256 / WAVE][WAVE];  // each wavefront's leaf sources (accelerations)
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   // the whole wavefront past the end (its first target index is uniform)
   if (blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~(WAVE - 1)) >= m) return;
@@ -139,10 +164,10 @@ __global__ __launch_bounds__(256) void k_walk(const NodeD* __restrict__ nodes,
           double px = 0.0, py = 0.0, pz = 0.0;
           bool redo = true;
           if (fast_eps) {
-            const bool bad = g_one ? leaf_mid<DIM, SELF, true>(spart, j0, j1, t, mine, tx, ty, tz, eps2, G, px,
-                                                               py, pz)
-                                   : leaf_mid<DIM, SELF, false>(spart, j0, j1, t, mine, tx, ty, tz, eps2, G, px,
-                                                                py, pz);
+            double4* wt = ltile[threadIdx.x / WAVE];
+            const bool bad =
+                g_one ? leaf_mid<DIM, SELF, true>(wt, spart, j0, j1, t, mine, tx, ty, tz, eps2, G, px, py, pz)
+                      : leaf_mid<DIM, SELF, false>(wt, spart, j0, j1, t, mine, tx, ty, tz, eps2, G, px, py, pz);
             redo = __ballot(bad) != 0;
           }
           bool zdiv = false;
