@@ -287,7 +287,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(74))) void k_wa
             if (DIM == 3) r2 = r2 + dz * dz;
             const double r2s = r2 + eps2;
             if (r2s != 0.0) {
-              const double f = (G * nd.m) / (r2s * sqrt(r2s));
+              // the middle-range square root and division where r2s and G m
+              // allow them (then r2s * sqrt(r2s) lies in [2^-350, 2^350])
+              const double gm = G * nd.m;
+              double f;
+              if (r2s >= 0x1p-233 && r2s <= 0x1p233 && gm >= 0x1p-300 && gm <= 0x1p300) {
+                f = div_mid(gm, r2s * sqrt_mid(r2s));
+              } else {
+                f = gm / (r2s * sqrt(r2s));
+              }
               ax = ax + f * dx;
               ay = ay + f * dy;
               if (DIM == 3) az = az + f * dz;
