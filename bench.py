@@ -362,6 +362,9 @@ def workload_plummer(args, env):
                 # independent FP32 FMAs alone (2 flops each): the vector ALU's
                 # issue ceiling on this device without packed instructions
                 roof["valu_fma_ceiling_tflops"] = 2 * ceil["fma_lane_per_s"] / 1e12
+            if "rsq_lane_per_s" in ceil and "fma_lane_per_s" in ceil:
+                # v_rsq_f32 issue cost in FMA issue slots (one per leaf pair)
+                roof["rsq_cost_in_fma_slots"] = ceil["fma_lane_per_s"] / ceil["rsq_lane_per_s"]
             print(f"bench: pair-loop ceiling {tf:.1f} TFLOP/s; evaluation at {roof['achieved'] / tf:.1%} of it",
                   file=sys.stderr)
     line["roofline"] = roof

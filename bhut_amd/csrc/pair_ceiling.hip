@@ -11,7 +11,8 @@
 // reciprocal square root by a multiply, to show its share of the loop;
 // "fma_lane_per_s" is the issue rate of independent FP32 FMAs alone (eight
 // chains per lane, same launch shape): the VALU ceiling on this device
-// without packed instructions, in lane-FMAs per second.
+// without packed instructions, in lane-FMAs per second; "rsq_lane_per_s" the
+// same for v_rsq_f32.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -85,6 +86,26 @@ __global__ __launch_bounds__(256, 8) void k_fma_chains(int reps, float* __restri
   out[gid] = s;  // keeps the chains live
 }
 
+// Eight independent v_rsq_f32 chains per lane: the transcendental's issue
+// rate alone.
+__global__ __launch_bounds__(256, 8) void k_rsq_chains(int reps, float* __restrict__ out) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  float a[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) a[j] = 1.f + 0.001f * (threadIdx.x + j);
+  for (int r = 0; r < reps; r++) {
+#pragma unroll 16
+    for (int k = 0; k < 64; k++) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) a[j] = __builtin_amdgcn_rsqf(a[j]);
+    }
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; j++) s += a[j];
+  out[gid] = s;  // keeps the chains live
+}
+
 int main() {
   int ncu = 0;
   CK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0));
@@ -129,8 +150,20 @@ int main() {
     CK(hipEventElapsedTime(&ms, a, b));
     fma_rate = (double)blocks * 256 * freps * 64 * 8 / (ms * 1e-3);
   }
-  printf("{\"uniform\": %.6e, \"mass\": %.6e, \"no_rsq\": %.6e, \"fma_lane_per_s\": %.6e, \"cus\": %d, "
-         "\"pairs_per_launch\": %.6e}\n", rate[0], rate[1], rate[2], fma_rate, ncu, pairs);
+  double rsq_rate = 0.0;
+  for (int it = 0; it < 3; it++) {
+    CK(hipEventRecord(a));
+    hipLaunchKernelGGL(k_rsq_chains, dim3(blocks), dim3(256), 0, 0, freps, o);
+    CK(hipGetLastError());
+    CK(hipEventRecord(b));
+    CK(hipEventSynchronize(b));
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, a, b));
+    rsq_rate = (double)blocks * 256 * freps * 64 * 8 / (ms * 1e-3);
+  }
+  printf("{\"uniform\": %.6e, \"mass\": %.6e, \"no_rsq\": %.6e, \"fma_lane_per_s\": %.6e, "
+         "\"rsq_lane_per_s\": %.6e, \"cus\": %d, \"pairs_per_launch\": %.6e}\n", rate[0], rate[1], rate[2],
+         fma_rate, rsq_rate, ncu, pairs);
   CK(hipFree(d));
   CK(hipFree(o));
   return 0;
