@@ -40,30 +40,38 @@ namespace bh {
 // the leaf with sqrt() and '/').  G1: G == 1, where G * m == m exactly.
 // The leaf's sources pass through the wavefront's LDS tile `wt`, 64 at a time
 // (one coalesced load per chunk instead of one scalar load, and its wait,
-// per source), and LEAF_READS of them are read from the tile before their
-// pairs are computed; every lane still adds its pairs in source order.
-constexpr int LEAF_READS = 4;
+// per source), and a few of them are read from the tile before their pairs
+// are computed; every lane still adds its pairs in source order.
+// External targets compute every pair on every lane, with a zero magnitude
+// on lanes that do not visit the leaf (whose partial sums are discarded),
+// which lets the compiler interleave consecutive pairs; self targets keep the
+// per-pair branch (and skip their own particle), which measured faster there
+// (the walk of external targets touches more scattered leaves and gains from
+// the overlap).
+template <bool SELF>
+constexpr int leaf_reads() { return SELF ? 4 : 2; }
 
 template <int DIM, bool SELF, bool G1>
 __device__ __forceinline__ bool leaf_mid(double4* __restrict__ wt, const double4* __restrict__ spart, int j0,
                                          int j1, int64_t t, bool mine, double tx, double ty, double tz, double eps2,
                                          double G, double& px, double& py, double& pz) {
+  constexpr int LEAF_READS = leaf_reads<SELF>();
   bool bad = false;
   auto pair = [&](const double4& q, int j) {
-    if (mine && !(SELF && j == t)) {
-      const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
-      double rsq = dx * dx;
-      rsq = rsq + dy * dy;
-      if (DIM == 3) rsq = rsq + dz * dz;
-      rsq = rsq + eps2;
-      bad |= !(rsq <= 0x1p700);
-      const double rinv = recip_mid(sqrt_mid(rsq));
-      const double ric = rinv * rinv * rinv;
-      const double mag = (G1 ? q.w : G * q.w) * ric;
-      px = px + mag * dx;
-      py = py + mag * dy;
-      if (DIM == 3) pz = pz + mag * dz;
-    }
+    const bool use = mine && !(SELF && j == t);
+    if (SELF && !use) return;
+    const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+    double rsq = dx * dx;
+    rsq = rsq + dy * dy;
+    if (DIM == 3) rsq = rsq + dz * dz;
+    rsq = rsq + eps2;
+    bad |= use && !(rsq <= 0x1p700);
+    const double rinv = recip_mid(sqrt_mid(rsq));
+    const double ric = rinv * rinv * rinv;
+    const double mag = use ? (G1 ? q.w : G * q.w) * ric : 0.0;
+    px = px + mag * dx;
+    py = py + mag * dy;
+    if (DIM == 3) pz = pz + mag * dz;
   };
   const int lane = threadIdx.x % WAVE;
   for (int c0 = j0; c0 < j1; c0 += WAVE) {
