@@ -428,3 +428,30 @@ def test_theta0_walk_bitwise_against_oracle(native, dim):
                    f"self eps {eps} G {G}")
         bits_equal(dt.accel_targets(q, 0.0, eps, G), O.accelerations(ot, x, m, q, softening=eps, theta=0.0, G=G),
                    f"targets eps {eps} G {G}")
+
+
+def test_external_targets_with_non_finite_rows(native):
+    """External targets are walked in wavefronts of 64 whose lanes all compute
+    every pair of a visited leaf; a target with a NaN or infinite coordinate
+    sharing a wavefront with finite ones leaves their results bit for bit
+    what they are without it (each lane's sums are its own), and its own row
+    follows the oracle (non-finite where the oracle's is)."""
+    rng = np.random.default_rng(77)
+    x = rng.standard_normal((20_000, 3))
+    m = rng.uniform(0.5, 1.5, 20_000)
+    q = rng.standard_normal((700, 3)) * 0.7
+    bad = [7, 100, 101, 650]
+    q[7, 0] = np.nan
+    q[100, 2] = np.inf
+    q[101] = -np.inf
+    q[650, 1] = np.nan
+    dt = native.DeviceTree.build(x, m, 32)
+    ot = O.build(x, m, 32)
+    good = np.setdiff1d(np.arange(len(q)), bad)
+    for eps, G in ((0.01, 1.0), (0.01, 3.0), (0.0, 1.0)):
+        got = dt.accel_targets(q, 0.5, eps, G)
+        alone = dt.accel_targets(q[good], 0.5, eps, G)
+        bits_equal(got[good], alone, f"finite rows, eps {eps} G {G}")
+        ref = O.accelerations(ot, x, m, q, softening=eps, theta=0.5, G=G)
+        acc_close(got[good], ref[good])
+        np.testing.assert_array_equal(np.isfinite(got[bad]), np.isfinite(ref[bad]))
