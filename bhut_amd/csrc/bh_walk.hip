@@ -24,10 +24,10 @@
 // depth d it is min(follow, d), and the lane visits the node iff follow == d;
 // opening the node sets follow = d + 1.  The wavefront goes down into the
 // children iff some lane opened the node, else on to `skip`.  Node records
-// and leaf particles are read once per wavefront (uniform scalar loads) and
-// broadcast to the lanes; a leaf's particle pairs run on every visiting lane
-// at once, where the per-lane walk of the same lists ran only the lanes that
-// happened to be at a leaf.
+// are read once per wavefront (uniform scalar loads) and leaf particles once
+// per wavefront through its LDS tile, and broadcast to the lanes; a leaf's
+// particle pairs run on every visiting lane at once, where a per-lane walk of
+// the same lists would run only the lanes that happened to be at a leaf.
 #include "bh_common.h"
 #include "bh_fp64.h"
 
