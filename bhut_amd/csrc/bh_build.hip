@@ -387,7 +387,12 @@ __device__ __forceinline__ void up_leaf(int v, const int32_t* __restrict__ start
 #pragma unroll
     for (int d = 0; d < DIM; d++) ctr[d] = center[v * DIM + d];
   }
-  const double total = 0.0 + (BIG ? np_pairwise_mass(p, c) : pw_block(p, c));
+  double total = 0.0;
+  if (BIG) {  // np.sum's buffers of 8192 elements, each summed pairwise
+    for (int64_t s0 = 0; s0 < c; s0 += 8192) total += np_pairwise_mass(p + s0, c - s0 < 8192 ? c - s0 : 8192);
+  } else {
+    total = 0.0 + pw_block(p, c);
+  }
   mass[v] = total;
   if (total > 0) {
     double sx[DIM];

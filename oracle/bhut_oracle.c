@@ -50,8 +50,14 @@ static double np_pairwise(const double *a, int64_t n) {
 }
 
 /* np.sum over a 1-D float64 array: the reduction starts from the additive
- * identity +0.0 and adds the pairwise sum of the whole run. */
-double orc_np_sum(const double *a, int64_t n) { return 0.0 + np_pairwise(a, n); }
+ * identity +0.0 and runs its inner loop over buffers of at most 8192
+ * elements, adding the pairwise sum of each buffer in turn (pinned against
+ * np.sum at lengths up to 250 001). */
+double orc_np_sum(const double *a, int64_t n) {
+  double acc = 0.0;
+  for (int64_t s = 0; s < n; s += 8192) acc += np_pairwise(a + s, n - s < 8192 ? n - s : 8192);
+  return acc;
+}
 
 /* np.min / np.max along axis 0: NaN propagates (bbox.py:43-44). */
 static double np_min2(double acc, double v) {

@@ -337,16 +337,22 @@ def test_edge_cases(native):
 
 
 def test_leaf_larger_than_pairwise_block(native):
-    """Leaves above 128 particles take NumPy's recursive pairwise split."""
+    """Leaves above 128 particles take NumPy's recursive pairwise split, and
+    leaves above 8192 np.sum's buffers of 8192 elements (leaf masses equal
+    np.sum of the leaf's sorted masses, bit for bit)."""
     rng = np.random.default_rng(6)
-    x = rng.standard_normal((3000, 3))
-    m = rng.uniform(0.0, 1.0, 3000) * 10.0 ** rng.integers(-6, 6, 3000)
-    for leaf in (200, 700):
-        dt = native.DeviceTree.build(x, m, leaf)
-        ot = O.build(x, m, leaf)
-        e = dt.export()
-        for f in ("mass", "com"):
-            bits_equal(e[f], getattr(ot, f), f"leaf {leaf} {f}")
+    for n, leaves in ((3000, (200, 700)), (30_000, (16_000, 30_000))):
+        x = rng.standard_normal((n, 3))
+        m = rng.uniform(0.0, 1.0, n) * 10.0 ** rng.integers(-6, 6, n)
+        for leaf in leaves:
+            dt = native.DeviceTree.build(x, m, leaf)
+            ot = O.build(x, m, leaf)
+            e = dt.export()
+            for f in ("mass", "com"):
+                bits_equal(e[f], getattr(ot, f), f"leaf {leaf} {f}")
+            for v in np.flatnonzero(e["is_leaf"]):
+                a, c = int(e["start"][v]), int(e["count"][v])
+                assert e["mass"][v] == np.sum(m[e["perm"][a:a + c]]), (leaf, v)
 
 
 def test_full_size_plummer_properties(native):

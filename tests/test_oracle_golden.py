@@ -90,9 +90,11 @@ def test_accelerations_and_lists_bit_exact(golden, eps):
 
 def test_np_sum_restatement_against_numpy():
     """orc_np_sum against np.sum itself on many lengths (pairwise blocks,
-    the 128 threshold, the recursive split) and magnitudes."""
+    the 128 threshold, the recursive split, the reduction's buffers of 8192
+    elements) and magnitudes."""
     rng = np.random.default_rng(5)
-    for n in list(range(1, 300)) + [511, 1000, 4097]:
+    for n in list(range(0, 300)) + [511, 1000, 4097, 8191, 8192, 8193, 9000, 10000, 12345, 16385, 20000, 24577,
+                                    50000, 250001]:
         a = rng.standard_normal(n) * 10.0 ** rng.integers(-6, 6, size=n)
         assert O.np_sum(a) == np.sum(a), n
 
